@@ -1,0 +1,102 @@
+/* qsf_hip.h — C ABI of libqsf_hip.so, the MI355X (gfx950) split-operator step of QSF.
+ *
+ * The reference has no runtime ABI: a user program instantiates header templates and the grid
+ * type provides the member-template surface that SplitPropagator calls. Each entry point below
+ * replaces one member of that surface (cited), so a grid backend (qsf_amd/host/qsf/device_grid.h)
+ * can forward SplitPropagator's calls here unchanged.
+ *
+ * Conventions: every function returns 0 on success and a negative code on error
+ * (qsf_last_error(ctx) returns the message); handles are owned by the library; host buffers are
+ * owned by the caller; wave functions cross the boundary as interleaved FP64 (re, im) in the
+ * reference's row-major layout (X rep) or FFTW's transposed P layout (D>1 main grids,
+ * wf/grid.h:238-248). One HIP stream per context; calls on one context are not thread-safe.
+ */
+#ifndef QSF_HIP_H
+#define QSF_HIP_H
+
+#include "qsf_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qsf_ctx qsf_ctx;
+typedef struct qsf_set qsf_set;
+typedef struct qsf_grid qsf_grid;
+typedef struct qsf_prop qsf_prop;
+
+enum {
+  QSF_OK = 0,
+  QSF_ERR_ARG = -1,        /* invalid argument                         */
+  QSF_ERR_UNSUPPORTED = -2,/* configuration the device path lacks      */
+  QSF_ERR_HIP = -3,        /* HIP runtime error                        */
+  QSF_ERR_STATE = -4       /* call not valid in the current state      */
+};
+
+/* ---- context (one device, one stream) ---- */
+int qsf_ctx_create(int device, qsf_ctx** out);
+int qsf_ctx_destroy(qsf_ctx* ctx);
+const char* qsf_last_error(const qsf_ctx* ctx);
+int qsf_ctx_synchronize(qsf_ctx* ctx);
+/* Device time (ms) between the two most recent qsf_timer_mark calls, on the context stream. */
+int qsf_timer_mark(qsf_ctx* ctx);
+int qsf_timer_elapsed_ms(qsf_ctx* ctx, double* ms);
+
+/* ---- grid sets: one grid, or the 2^D regions of MultiRegionsReduced (mpi_logic.h:246-263) ---- */
+/* Replaces LocalGrid(Section&) construction (grid.h:115-123, multigrid.h:45-46, 124-202). */
+int qsf_set_create(qsf_ctx* ctx, const qsf_grid_desc* desc, qsf_set** out);
+int qsf_set_destroy(qsf_set* set);
+int qsf_set_count(const qsf_set* set, int* count);
+int qsf_set_grid(qsf_set* set, int region, qsf_grid** out);
+/* LocalGrid<...,true>::postCompute: transfer() then maskRegion() (multigrid.h:617-657). */
+int qsf_set_post_compute(qsf_set* set);
+
+/* ---- one grid: the HamWF member surface SplitPropagator calls ---- */
+int qsf_grid_size(const qsf_grid* g, int64_t* m);                       /* localSize grid.h:359 */
+int qsf_grid_rep(const qsf_grid* g, int* rep);
+/* operator[] access in bulk (grid.h:296). rep = representation the data is in. */
+int qsf_grid_upload(qsf_grid* g, const double* interleaved, int64_t count, int rep);
+int qsf_grid_download(qsf_grid* g, double* interleaved, int64_t count);
+/* fourier<REP>() grid.h:369-378 (main) / multigrid.h:666-680 (partial FFT of regions). */
+int qsf_fourier(qsf_grid* g, int to_rep);
+/* evolve<MODE,REP>(delta) grid.h:394-423 with Spin0 hamiltonian.h:44-97. coupling3 holds
+ * _coupling[Axis<d>] (coupling.h:63-67); couples_in = couplesInRep (QSF_REP_X | QSF_REP_P). */
+int qsf_evolve(qsf_grid* g, int mode, int rep, double delta, const double* coupling3, int couples_in);
+/* average<REP,Op>() grid.h:425-441; op 0 = Identity, 1 = PotentialEnergy, 2 = KineticEnergy. */
+int qsf_average(qsf_grid* g, int rep, int op, double* out);
+/* operation<M,R,Normalize>() grid.h:599-606. */
+int qsf_normalize(qsf_grid* g);
+/* current_map() + velocityGaugeCorrection + flux<R,F>() (grid.h:467-573, propagator.h:249-265).
+ * vg_coupling3 = A per axis for the velocity gauge, or NULL. */
+int qsf_flux(qsf_grid* g, const double* vg_coupling3, const qsf_border_desc* borders, int nb,
+             double* out);
+/* maskRegion() grid.h:582-597 (bit-exact: host-built FP64 mask table). */
+int qsf_mask_region(qsf_grid* g);
+
+/* ---- propagator: SplitPropagator<MODE, MultiProductSplit<Split3Base<X,P,X>,order>, ...>
+ *      (propagator.h:51-499) over a grid set, with the step executed as fused axis passes ---- */
+int qsf_prop_create(qsf_set* set, const qsf_prop_desc* desc, const qsf_field_desc* fields, int nfields,
+                    qsf_prop** out);
+int qsf_prop_destroy(qsf_prop* p);
+int qsf_prop_max_steps(const qsf_prop* p, int64_t* out);      /* propagator.h:153 */
+int qsf_prop_start(qsf_prop* p);                              /* computeEach<AT_START> :439-441 */
+int qsf_prop_should_continue(qsf_prop* p, int* out);          /* shouldContinue :304-309 */
+int qsf_prop_step(qsf_prop* p);                               /* one loop body :445-449 */
+int qsf_prop_run_steps(qsf_prop* p, int64_t k, int64_t* done); /* up to k loop bodies, no host sync */
+int qsf_prop_finish(qsf_prop* p);                             /* final step + AT_END :451-454 */
+int qsf_prop_step_count(const qsf_prop* p, int64_t* out);
+int qsf_prop_time(const qsf_prop* p, double* out);
+int qsf_prop_coupling(const qsf_prop* p, double* out3);
+/* Observable rows (one per computeEach): step, time, norm, epot, ekin, etot, ediff, flux[nb]. */
+int qsf_prop_record_width(const qsf_prop* p, int* out);
+int qsf_prop_nrecords(qsf_prop* p, int* out);
+int qsf_prop_records(qsf_prop* p, double* out, int max_rows);
+/* FP64 time integral of each flux family, sum over steps of flux*dt (device-resident). */
+int qsf_prop_flux_integral(qsf_prop* p, double* out);
+/* 0 = fused axis passes (default), 1 = one standalone call per reference member. */
+int qsf_prop_set_exec(qsf_prop* p, int exec_mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

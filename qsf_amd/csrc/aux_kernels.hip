@@ -1,0 +1,294 @@
+// aux_kernels.hip — small kernels around the axis passes: per-substep phase tables, pair
+// tables, FFT roots, deterministic reductions, the surface-flux reduction and conversions.
+#include <hip/hip_runtime.h>
+
+#include "../../include/qsf_types.h"
+#include "fft_core.hpp"
+#include "qsf_launch.hpp"
+
+namespace qsf {
+
+struct AxisTableBatch {
+  AxisTableArgs a[12];
+};
+
+template <typename Real>
+__global__ void axis_tables_kernel(AxisTableBatch batch) {
+  using T = typename cvec<Real>::T;
+  const AxisTableArgs& a = batch.a[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double h;
+  if (a.kind == 0) {  // X: potential (per-axis part) + x·F  (hamiltonian.h:44-79)
+    const double x = a.xmin + i * a.dx;
+    const double ne = a.use_ne ? a.NE / sqrt(x * x + a.Nsoft) : 0.0;
+    h = ne + x * a.C;
+  } else {            // P: ½p² + p·A
+    const double p = a.dp * (double)(i < a.pn2 ? i : i - a.pn);
+    h = (0.5 * p * p) + (p * a.C);
+  }
+  const double v = a.delta * h;
+  double re, im;
+  if (a.mode == QSF_MODE_IM) { re = exp(-v); im = 0.0; }
+  else { sincos(-v, &im, &re); }
+  if (a.kind == 0 && a.apply_abs && a.absorber != QSF_ABS_NONE) {
+    const int64_t half = a.n / 2;
+    const int64_t nd = i < half ? -i : i - a.n + 1;   // neg_dist_from_edge grid.h:181-185
+    const int64_t node = a.nCAP + nd;
+    if (node > 0) {
+      double f;
+      if (a.absorber == QSF_ABS_CAP) {
+        const double s4 = (double)(node * (node * (node * node)));
+        f = exp(-a.delta * s4 * a.eta);                  // absorbers.h:40-46, one axis
+      } else {
+        const double c = cos(0.5 * M_PI * (double)node / (double)a.nCAP);
+        f = pow(pow(c, 0.125), a.delta / a.tau);         // cos^(1/8) per tau of time
+      }
+      re *= f;
+      im *= f;
+    }
+  }
+  re *= a.norm;
+  im *= a.norm;
+  reinterpret_cast<T*>(a.out)[i] = T{(Real)re, (Real)im};
+}
+
+void launch_axis_tables(int precision, const AxisTableArgs* a, int count, hipStream_t s) {
+  for (int off = 0; off < count; off += 12) {
+    AxisTableBatch b;
+    int k = count - off < 12 ? count - off : 12;
+    int64_t nmax = 0;
+    for (int i = 0; i < k; ++i) {
+      b.a[i] = a[off + i];
+      nmax = a[off + i].n > nmax ? a[off + i].n : nmax;
+    }
+    dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)k);
+    if (precision == QSF_FP64) hipLaunchKernelGGL(axis_tables_kernel<double>, grid, dim3(256), 0, s, b);
+    else hipLaunchKernelGGL(axis_tables_kernel<float>, grid, dim3(256), 0, s, b);
+  }
+}
+
+template <typename Real>
+__global__ void pair_table_kernel(PairTableArgs a) {
+  using T = typename cvec<Real>::T;
+  const int64_t total = a.kind == 1 ? a.n : a.n * a.n;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total) return;
+  double ee;
+  if (a.kind == 1) {  // |i-j| = k (Eberly: depends on x-y only)
+    const double d = (double)k * a.dx;
+    ee = a.EE / sqrt(d * d + a.Esoft);
+  } else {
+    const int64_t i = k / a.n, j = k % a.n;
+    const double x = a.xmin + i * a.dx, y = a.xmin + j * a.dx;
+    if (a.model == QSF_POT_ECKHARDT_SACHA) ee = a.EE / sqrt((x - y) * (x - y) + x * y + a.Esoft);
+    else ee = a.EE / sqrt((x - y) * (x - y) + a.Esoft);
+  }
+  const double v = a.delta * ee;
+  double re, im;
+  if (a.mode == QSF_MODE_IM) { re = exp(-v); im = 0.0; }
+  else { sincos(-v, &im, &re); }
+  reinterpret_cast<T*>(a.out)[k] = T{(Real)re, (Real)im};
+}
+
+void launch_pair_table(int precision, const PairTableArgs& a, hipStream_t s) {
+  const int64_t total = a.kind == 1 ? a.n : a.n * a.n;
+  dim3 grid((unsigned)((total + 255) / 256));
+  if (precision == QSF_FP64) hipLaunchKernelGGL(pair_table_kernel<double>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(pair_table_kernel<float>, grid, dim3(256), 0, s, a);
+}
+
+template <typename Real>
+__global__ void roots_kernel(typename cvec<Real>::T* out, int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double s, c;
+  sincospi(-2.0 * (double)j / (double)n, &s, &c);
+  out[j] = typename cvec<Real>::T{(Real)c, (Real)s};
+}
+
+void launch_roots(int precision, void* out, int64_t n, hipStream_t s) {
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (precision == QSF_FP64)
+    hipLaunchKernelGGL(roots_kernel<double>, grid, dim3(256), 0, s, reinterpret_cast<double2*>(out), n);
+  else
+    hipLaunchKernelGGL(roots_kernel<float>, grid, dim3(256), 0, s, reinterpret_cast<float2*>(out), n);
+}
+
+// One block, fixed summation order: thread k sums blocks k, k+256, ... then a tree.
+__global__ void reduce_partials_kernel(const double* partials, int64_t count, int width, double* out) {
+  __shared__ double red[256];
+  for (int c = 0; c < width; ++c) {
+    double acc = 0.0;
+    for (int64_t k = threadIdx.x; k < count; k += 256) acc += partials[k * width + c];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+    __syncthreads();
+  }
+}
+
+void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, count, width, out);
+}
+
+template <typename Real>
+__global__ void flux_kernel(FluxArgs a) {
+  using T = typename cvec<Real>::T;
+  const T* g = reinterpret_cast<const T*>(a.gathered);
+  const int D = a.D, nb = a.nb, stride = 1 + 2 * D;
+  double acc[16];
+  for (int b = 0; b < nb; ++b) acc[b] = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.npts;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* q = a.pts + p * stride;
+    const T c = g[q[0]];
+    const double cr = c.x, ci = c.y, nrm = cr * cr + ci * ci;
+    double j[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+      const int32_t sp = q[1 + d], sm = q[1 + D + d];
+      if (sp >= 0 && sm >= 0) {
+        // inv_2dx * Im(conj(ψ_c) (ψ_+ - ψ_-))  (grid.h:467-481)
+        const double dr = (double)g[sp].x - (double)g[sm].x, di = (double)g[sp].y - (double)g[sm].y;
+        j[d] = a.inv_2dx[d] * (cr * di - ci * dr);
+      }
+      j[d] += a.vgC[d] * nrm;  // velocityGaugeCorrection (propagator.h:249-257)
+    }
+    const double* bv = a.bvec + p * nb * D;
+    for (int b = 0; b < nb; ++b) {
+      double s = 0.0;
+      for (int d = 0; d < D; ++d) s += j[d] * bv[b * D + d];
+      acc[b] += s;
+    }
+  }
+  __shared__ double red[256];
+  for (int b = 0; b < nb; ++b) {
+    red[threadIdx.x] = acc[b];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partials[blockIdx.x * nb + b] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void flux_finish_kernel(const double* partials, int64_t count, int nb, double scale, double* out,
+                                   double* accum, double w) {
+  if (threadIdx.x != 0) return;
+  for (int b = 0; b < nb; ++b) {
+    double s = 0.0;
+    for (int64_t k = 0; k < count; ++k) s += partials[k * nb + b];
+    out[b] = s * scale;
+    if (accum) accum[b] += out[b] * w;
+  }
+}
+
+void launch_flux(const FluxArgs& a, hipStream_t s) {
+  const int blocks = 64;
+  if (a.precision == QSF_FP64) hipLaunchKernelGGL(flux_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(flux_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(flux_finish_kernel, dim3(1), dim3(64), 0, s, a.partials, (int64_t)blocks, a.nb, a.scale,
+                     a.out, a.accum, a.accum_weight);
+}
+
+template <typename Real>
+__global__ void scale_kernel(typename cvec<Real>::T* psi, int64_t m, double s) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    auto v = psi[i];
+    psi[i] = typename cvec<Real>::T{(Real)(v.x * s), (Real)(v.y * s)};
+  }
+}
+
+void launch_scale(int precision, void* psi, int64_t m, double s, hipStream_t st) {
+  const int blocks = (int)((m + 255) / 256 < 8192 ? (m + 255) / 256 : 8192);
+  if (precision == QSF_FP64)
+    hipLaunchKernelGGL(scale_kernel<double>, dim3(blocks), dim3(256), 0, st, reinterpret_cast<double2*>(psi), m, s);
+  else
+    hipLaunchKernelGGL(scale_kernel<float>, dim3(blocks), dim3(256), 0, st, reinterpret_cast<float2*>(psi), m, s);
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(const A* src, B* dst, int64_t m) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+    A v = src[i];
+    dst[i] = B{(decltype(dst->x))v.x, (decltype(dst->x))v.y};
+  }
+}
+
+void launch_copy_convert(int from_prec, const void* src, int to_prec, void* dst, int64_t m, hipStream_t s) {
+  const int blocks = (int)((m + 255) / 256 < 8192 ? (m + 255) / 256 : 8192);
+  if (from_prec == QSF_FP64 && to_prec == QSF_FP32_MIXED)
+    hipLaunchKernelGGL((convert_kernel<double2, float2>), dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<const double2*>(src), reinterpret_cast<float2*>(dst), m);
+  else if (from_prec == QSF_FP32_MIXED && to_prec == QSF_FP64)
+    hipLaunchKernelGGL((convert_kernel<float2, double2>), dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<const float2*>(src), reinterpret_cast<double2*>(dst), m);
+}
+
+}  // namespace qsf
+
+namespace qsf {
+
+template <typename Real>
+__global__ void slice_2d_kernel(SliceArgs a) {
+  using T = typename cvec<Real>::T;
+  const int64_t m = a.n0 * a.n1;
+  const T* src = reinterpret_cast<const T*>(a.src);
+  T* out = reinterpret_cast<T*>(a.slice);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i0 = k / a.n1, i1 = k % a.n1;
+    const int64_t i_f = a.dirFree == 0 ? i0 : i1, n_f = a.dirFree == 0 ? a.n0 : a.n1;
+    const int64_t i_o = a.dirFree == 0 ? i1 : i0, n_o = a.dirFree == 0 ? a.n1 : a.n0;
+    // edge rows/columns of the free axis (getBox multigrid.h:454-482)
+    const bool edge = i_f < a.nCAP || i_f >= n_f - a.nCAP;
+    T v{Real(0), Real(0)};
+    if (edge) {
+      const int64_t nd = i_f < n_f / 2 ? -i_f : i_f - n_f + 1;   // neg_dist_from_edge
+      int64_t node = a.nCAP + nd;
+      node = node < 0 ? 0 : node;
+      double w = a.inv_tab[node];
+      if (a.corr_tab) {
+        const int64_t ndo = i_o < n_o / 2 ? -i_o : i_o - n_o + 1;
+        int64_t nodo = a.nCAP + ndo;
+        nodo = nodo < 0 ? 0 : nodo;
+        w = w * a.corr_tab[nodo];                                  // maskSlice :547-548
+      }
+      const T x = src[k];
+      v = T{(Real)((double)x.x * w), (Real)((double)x.y * w)};
+    }
+    out[k] = v;
+  }
+}
+
+void launch_slice_2d(int precision, const SliceArgs& a, hipStream_t s) {
+  const int64_t m = a.n0 * a.n1;
+  const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
+  if (precision == QSF_FP64) hipLaunchKernelGGL(slice_2d_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(slice_2d_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <typename T>
+__global__ void add_kernel(T* dst, const T* src, int64_t m) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    T a = dst[k], b = src[k];
+    dst[k] = T{a.x + b.x, a.y + b.y};
+  }
+}
+
+void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s) {
+  const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
+  if (precision == QSF_FP64)
+    hipLaunchKernelGGL(add_kernel<double2>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<double2*>(dst),
+                       reinterpret_cast<const double2*>(src), m);
+  else
+    hipLaunchKernelGGL(add_kernel<float2>, dim3(blocks), dim3(256), 0, s, reinterpret_cast<float2*>(dst),
+                       reinterpret_cast<const float2*>(src), m);
+}
+
+}  // namespace qsf

@@ -1,0 +1,391 @@
+#pragma once
+// axis_pass_impl.hpp — the fused axis pass of the split-operator step (gfx950).
+//
+// One launch sweeps the whole grid once along one axis. A block owns W transforms of length
+// N: W consecutive rows (row mode: the contiguous last axis) or W adjacent columns (column
+// mode: a strided axis, so every global access is a W*sizeof(complex) contiguous segment).
+// Per transform it can do, in registers: inverse FFT -> X phase (A half) -> observables of the
+// step-boundary state -> X phase (B half) -> forward FFT -> P phase -> inverse FFT. This is
+// the reference's chainEvolve sequence (propagator.h:311-332: fourier + evolve_ grid.h:394-418)
+// with the X halves of consecutive substeps meeting inside one pass.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "fft_core.hpp"
+#include "pass_params.hpp"
+#include "qsf_launch.hpp"
+
+namespace qsf {
+
+template <typename Real> struct Tr {
+  using T = typename cvec<Real>::T;
+};
+
+// Block geometry per (precision, N, mode).
+template <typename Real, int N, bool ROW> struct Geo {
+  using PL = PlanOf<N>;
+  static constexpr int T = PL::T;
+  static constexpr int CB = (int)sizeof(typename cvec<Real>::T);
+  static constexpr int lds_cap(int w) { return w * N * CB; }
+  static constexpr int pick() {
+    if (ROW) {
+      int w = 256 / T;
+      if (w < 1) w = 1;
+      while (w > 1 && lds_cap(w) > 65536) w /= 2;
+      return w;
+    }
+    int w = 16;
+    const int budget = (N * CB >= 32768) ? 131072 : 65536;
+    while (w > 1 && (lds_cap(w) > budget || w * T > 1024)) w /= 2;
+    return w;
+  }
+  static constexpr int W = pick();
+  static constexpr int THREADS = W * T;
+};
+
+template <typename T> struct RowAddr {
+  int base;
+  __device__ __forceinline__ int operator()(int p) const { return base + p; }
+};
+template <int W> struct ColAddr {
+  int w;
+  __device__ __forceinline__ int operator()(int p) const { return p * W + w; }
+};
+
+template <typename T>
+__device__ __forceinline__ T ld(const void* tab, int64_t i) {
+  return reinterpret_cast<const T*>(tab)[i];
+}
+
+// Per-thread evaluator of an X factor for the elements of one transform along `axis`.
+template <typename T, int N> struct XEval {
+  T cst;
+  const T* ax;
+  const T* pr[2];
+  int other[2];
+  int npr;
+  int kind;
+  __device__ void init(const XFactor& f, int axis, const int* idx) {
+    using R = decltype(cst.x);
+    cst = T{R(1), R(0)};
+    ax = reinterpret_cast<const T*>(f.axis[axis]);
+    for (int d = 0; d < 3; ++d)
+      if (d != axis && f.axis[d]) cst = cmul(cst, ld<T>(f.axis[d], idx[d]));
+    npr = 0;
+    kind = f.pair_kind;
+    if (kind != PAIR_NONE) {
+      const int P[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+      for (int k = 0; k < 3; ++k) {
+        if (!(f.pair_mask & (1 << k))) continue;
+        int p = P[k][0], q = P[k][1];
+        if (p != axis && q != axis) {
+          int64_t i = kind == PAIR_DIFF ? (int64_t)abs(idx[p] - idx[q]) : (int64_t)idx[p] * f.pair_n + idx[q];
+          cst = cmul(cst, ld<T>(f.pair, i));
+        } else {
+          int o = (p == axis) ? idx[q] : idx[p];
+          pr[npr] = reinterpret_cast<const T*>(f.pair) + (kind == PAIR_FULL ? (int64_t)o * f.pair_n : 0);
+          other[npr] = o;
+          ++npr;
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ T at(int n) const {
+    T r = cst;
+    if (ax) r = cmul(r, ax[n]);
+    for (int k = 0; k < npr; ++k) {
+      T p = (kind == PAIR_DIFF) ? pr[k][abs(n - other[k])] : pr[k][n];
+      r = cmul(r, p);
+    }
+    return r;
+  }
+};
+
+__device__ __forceinline__ double pot_value(const ObsParams& o, const double* c, int D) {
+  // closed forms of potential.h (full or partial by free axes), evaluated in FP64
+  auto Ne = [&](double x) { return o.NE / sqrt(x * x + o.Nsoft); };
+  auto ee = [&](double x, double y) {
+    if (o.pot_kind == 3) return o.EE / sqrt((x - y) * (x - y) + x * y + o.Esoft);
+    return o.EE / sqrt((x - y) * (x - y) + o.Esoft);
+  };
+  const int fm = o.free_mask;
+  auto b = [&](int d) { return !(fm & (1 << d)); };
+  if (o.pot_kind == 0) return 0.0;
+  if (o.pot_kind == 1) {  // Coulomb over the bounded coordinates
+    double s = 0.0;
+    for (int d = D - 1; d >= 0; --d)
+      if (b(d)) s = c[d] * c[d] + s;
+    bool any = false;
+    for (int d = 0; d < D; ++d) any = any || b(d);
+    return any ? o.NE / sqrt(s + o.Nsoft) : 0.0;
+  }
+  double v = 0.0;
+  if (D == 1) return b(0) ? Ne(c[0]) : 0.0;
+  if (D == 2) {
+    if (b(0) && b(1)) v = ee(c[0], c[1]);
+    if (b(0)) v += Ne(c[0]);
+    if (b(1)) v += Ne(c[1]);
+    return v;
+  }
+  if (b(0) && b(1)) v += ee(c[0], c[1]);
+  if (b(1) && b(2)) v += ee(c[1], c[2]);
+  if (b(2) && b(0)) v += ee(c[2], c[0]);
+  for (int d = 0; d < 3; ++d)
+    if (b(d)) v += Ne(c[d]);
+  return v;
+}
+
+template <typename Real, int N, bool ROW>
+__global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
+    axis_pass_kernel(PassParams p) {
+  using T = typename Tr<Real>::T;
+  using PL = PlanOf<N>;
+  constexpr int E = PL::E, TT = PL::T;
+  constexpr int W = Geo<Real, N, ROW>::W;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+
+  const int tid = threadIdx.x;
+  const int t = ROW ? (tid % TT) : (tid / W);
+  const int w = ROW ? (tid / TT) : (tid % W);
+
+  // XCD-aware tile order: consecutive tiles land on the same XCD (blocks go round-robin).
+  int64_t b = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+
+  // tile -> (outer o, inner c) of this thread's transform
+  int64_t o, c;
+  bool valid;
+  if (ROW) {
+    o = b * W + w;
+    c = 0;
+    valid = o < p.outer;
+  } else {
+    const int64_t cpt = (p.inner + W - 1) / W;  // column tiles per outer index
+    o = b / cpt;
+    c = (b % cpt) * W + w;
+    valid = (o < p.outer) && (c < p.inner);
+  }
+  const int64_t base = ROW ? o * N : o * (int64_t)N * p.inner + c;
+  const int64_t step = ROW ? 1 : p.inner;
+
+  int idx[3] = {0, 0, 0};
+  if (p.axis == 0) {
+    idx[1] = (int)(c / p.n[2]);
+    idx[2] = (int)(c % p.n[2]);
+  } else if (p.axis == 1) {
+    idx[0] = (int)o;
+    idx[2] = (int)c;
+  } else {
+    idx[0] = (int)(o / p.n[1]);
+    idx[1] = (int)(o % p.n[1]);
+  }
+
+  const T* roots = reinterpret_cast<const T*>(p.roots);
+  const T* src = reinterpret_cast<const T*>(p.src);
+  T* dst = reinterpret_cast<T*>(p.dst);
+
+  T v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = valid ? src[base + (int64_t)(t + TT * q) * step] : T{Real(0), Real(0)};
+  if (p.flags & F_SCALE) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], Real(p.scale));
+  }
+
+  auto run_fft = [&](auto dir_tag) {
+    constexpr int DIR = decltype(dir_tag)::value;
+    if constexpr (ROW) {
+      RowAddr<T> a{w * N};
+      fft<N, DIR>(v, t, lds, a, roots);
+    } else {
+      ColAddr<W> a{w};
+      fft<N, DIR>(v, t, lds, a, roots);
+    }
+  };
+  using FWD = std::integral_constant<int, -1>;
+  using INV = std::integral_constant<int, +1>;
+
+  if (p.flags & F_INV1) run_fft(INV{});
+
+  if (p.flags & F_XA) {
+    XEval<T, N> xe;
+    idx[p.axis] = 0;
+    xe.init(p.xa, p.axis, idx);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cmul(v[q], xe.at(t + TT * q));
+  }
+
+  if (p.flags & F_OBS) {
+    const ObsParams& ob = p.obs;
+    double nrm = 0.0, ep = 0.0, ek = 0.0;
+    if (valid) {
+      double co[3];
+      const int D = p.n[2] > 1 ? 3 : (p.n[1] > 1 ? 2 : 1);
+      for (int d = 0; d < 3; ++d) co[d] = ob.xmin[d] + idx[d] * ob.dx[d];
+      double kc = 0.0;
+      for (int d = 0; d < 3; ++d)
+        if (d != p.axis && ob.kin_w[d]) kc += ob.kin_w[d][idx[d]];
+      const double* ka = ob.kin_w[p.axis];
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        double a2 = (double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y;
+        nrm += a2;
+        if (ob.want_epot) {
+          co[p.axis] = ob.xmin[p.axis] + (t + TT * q) * ob.dx[p.axis];
+          ep += a2 * pot_value(ob, co, D);
+        }
+        if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[t + TT * q] : 0.0));
+      }
+      if (ob.copy_out) {
+        T* cp = reinterpret_cast<T*>(ob.copy_out);
+#pragma unroll
+        for (int q = 0; q < E; ++q) cp[base + (int64_t)(t + TT * q) * step] = v[q];
+      }
+    }
+    // block reduction (fixed order: lanes, then waves)
+    for (int off = 32; off > 0; off >>= 1) {
+      nrm += __shfl_xor(nrm, off);
+      ep += __shfl_xor(ep, off);
+      ek += __shfl_xor(ek, off);
+    }
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(smem_raw);
+    const int wave = tid >> 6, nwaves = (blockDim.x + 63) >> 6;
+    if ((tid & 63) == 0) { red[3 * wave] = nrm; red[3 * wave + 1] = ep; red[3 * wave + 2] = ek; }
+    __syncthreads();
+    if (tid == 0 && ob.partials) {
+      double a = 0.0, e2 = 0.0, e3 = 0.0;
+      for (int k = 0; k < nwaves; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
+      ob.partials[3 * blockIdx.x] = a;
+      ob.partials[3 * blockIdx.x + 1] = e2;
+      ob.partials[3 * blockIdx.x + 2] = e3;
+    }
+    __syncthreads();
+    if constexpr (ROW) {
+      if (ob.gather_rowptr) {
+        RowAddr<T> a{w * N};
+#pragma unroll
+        for (int q = 0; q < E; ++q) lds[a(t + TT * q)] = v[q];
+        __syncthreads();
+        if (valid) {
+          const int k0 = ob.gather_rowptr[o], k1 = ob.gather_rowptr[o + 1];
+          T* go = reinterpret_cast<T*>(ob.gather_out);
+          for (int k = k0 + t; k < k1; k += TT) go[k] = lds[a(ob.gather_pos[k])];
+        }
+        __syncthreads();
+      }
+    }
+  }
+
+  if (p.flags & F_MASK) {
+    // maskRegion (grid.h:582-597): one exact FP64 factor per point, from per-axis CAP nodes
+    int64_t mo = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (d == p.axis || p.mask_stride[d] == 0) continue;
+      const int64_t h = p.n[d] / 2, i = idx[d];
+      const int64_t nd = i < h ? -i : i - p.n[d] + 1;
+      int64_t node = p.nCAP + nd;
+      node = node < 0 ? 0 : node;
+      mo += node * p.mask_stride[d];
+    }
+    const int64_t ha = p.n[p.axis] / 2;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      int64_t off = mo;
+      if (p.mask_stride[p.axis]) {
+        const int64_t i = t + TT * q;
+        const int64_t nd = i < ha ? -i : i - p.n[p.axis] + 1;
+        int64_t node = p.nCAP + nd;
+        node = node < 0 ? 0 : node;
+        off += node * p.mask_stride[p.axis];
+      }
+      const double mk = p.mask_tab[off];
+      v[q] = T{(Real)((double)v[q].x * mk), (Real)((double)v[q].y * mk)};
+    }
+  }
+
+  if (p.flags & F_XB) {
+    XEval<T, N> xe;
+    idx[p.axis] = 0;
+    xe.init(p.xb, p.axis, idx);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cmul(v[q], xe.at(t + TT * q));
+  }
+
+  if (p.flags & F_FWD) run_fft(FWD{});
+
+  if (p.flags & F_P) {
+    T cst{Real(1), Real(0)};
+    for (int d = 0; d < 3; ++d)
+      if (d != p.axis && p.pf[d]) cst = cmul(cst, ld<T>(p.pf[d], idx[d]));
+    const T* pa = reinterpret_cast<const T*>(p.pf[p.axis]);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      T f = pa ? cmul(cst, pa[t + TT * q]) : cst;
+      v[q] = cmul(v[q], f);
+    }
+  }
+
+  if (p.flags & F_INV2) run_fft(INV{});
+
+  if (valid && !(p.flags & F_NOSTORE)) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) dst[base + (int64_t)(t + TT * q) * step] = v[q];
+  }
+}
+
+
+template <typename Real, int N, bool ROW>
+inline int launch_one(const PassParams& p, hipStream_t s) {
+  using G = Geo<Real, N, ROW>;
+  int64_t tiles;
+  if (ROW) tiles = (p.outer + G::W - 1) / G::W;
+  else tiles = p.outer * ((p.inner + G::W - 1) / G::W);
+  size_t lds = (size_t)G::W * N * sizeof(typename cvec<Real>::T);
+  if (lds < 16 * 3 * sizeof(double)) lds = 16 * 3 * sizeof(double);
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW>), dim3((unsigned)tiles), dim3(G::THREADS), lds, s, p);
+  return (int)tiles;
+}
+
+
+template <typename Real, bool ROW>
+inline int dispatch_n(int64_t n, const PassParams& p, hipStream_t s) {
+  switch (n) {
+#define QSF_CASE(NN) case NN: return launch_one<Real, NN, ROW>(p, s);
+    QSF_CASE(8) QSF_CASE(16) QSF_CASE(32) QSF_CASE(64) QSF_CASE(128) QSF_CASE(256)
+    QSF_CASE(512) QSF_CASE(1024) QSF_CASE(2048) QSF_CASE(4096)
+    QSF_CASE(12) QSF_CASE(24) QSF_CASE(48) QSF_CASE(96) QSF_CASE(192) QSF_CASE(384)
+    QSF_CASE(768) QSF_CASE(1536) QSF_CASE(3072)
+#undef QSF_CASE
+    default: return -1;
+  }
+}
+
+
+// Number of blocks a pass launches (sizes the partial-sum buffers).
+template <typename Real, bool ROW>
+inline int64_t tiles_n(int64_t n, const PassParams& p) {
+  int W = 1;
+  switch (n) {
+#define QSF_W(NN) case NN: W = Geo<Real, NN, ROW>::W; break;
+    QSF_W(8) QSF_W(16) QSF_W(32) QSF_W(64) QSF_W(128) QSF_W(256) QSF_W(512) QSF_W(1024)
+    QSF_W(2048) QSF_W(4096) QSF_W(12) QSF_W(24) QSF_W(48) QSF_W(96) QSF_W(192) QSF_W(384)
+    QSF_W(768) QSF_W(1536) QSF_W(3072)
+#undef QSF_W
+    default: return -1;
+  }
+  if (ROW) return (p.outer + W - 1) / W;
+  return p.outer * ((p.inner + W - 1) / W);
+}
+
+
+}  // namespace qsf

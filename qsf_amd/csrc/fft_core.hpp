@@ -1,0 +1,233 @@
+// fft_core.hpp — register-resident Stockham FFT building blocks for gfx950 (CDNA4).
+//
+// One length-N transform is spread over T = N/E threads; thread t holds the E elements
+// x[t + T*q], q = 0..E-1. With that assignment every Stockham stage reads its inputs in the
+// same pattern, and the last stage writes its outputs back into the same registers, so a
+// transform costs (stages - 1) LDS exchanges and its global loads/stores are coalesced
+// (consecutive threads touch consecutive elements of a row, or consecutive columns of a tile).
+//
+// The forward transform is e^{-2πi jk/N}, unnormalised, as FFTW_FORWARD in the reference
+// (wf/grid.h:251-274); the inverse is e^{+2πi jk/N}, unnormalised here (the 1/N is folded into
+// the phase tables by the caller).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qsf {
+
+template <typename R> struct cvec;
+template <> struct cvec<double> { using T = double2; };
+template <> struct cvec<float> { using T = float2; };
+
+template <typename T> __device__ __forceinline__ T cadd(T a, T b) { return T{a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ T csub(T a, T b) { return T{a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ T cmul(T a, T b) {
+  return T{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ T cconj(T a) { return T{a.x, -a.y}; }
+template <typename T, typename R> __device__ __forceinline__ T cscale(T a, R s) { return T{a.x * s, a.y * s}; }
+// multiply by DIR*i
+template <int DIR, typename T> __device__ __forceinline__ T mul_di(T a) {
+  return DIR < 0 ? T{a.y, -a.x} : T{-a.y, a.x};
+}
+
+// ---------------------------------------------------------------- small DFTs (in registers)
+template <int DIR, typename T> __device__ __forceinline__ void dft2(T& a, T& b) {
+  T t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void dft3(T& a, T& b, T& c) {
+  using R = decltype(a.x);
+  const R h = R(0.5);
+  const R s = R(0.86602540378443864676372317075294);  // sqrt(3)/2
+  T t1 = cadd(b, c), t2 = csub(b, c);
+  T m = T{a.x - h * t1.x, a.y - h * t1.y};
+  T r = mul_di<DIR>(cscale(t2, s));
+  a = cadd(a, t1);
+  b = cadd(m, r);
+  c = csub(m, r);
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void dft4(T& a, T& b, T& c, T& d) {
+  T s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = mul_di<DIR>(csub(b, d));
+  a = cadd(s0, s2);
+  c = csub(s0, s2);
+  b = cadd(s1, s3);
+  d = csub(s1, s3);
+}
+
+// w8^k for k = 1..3 with w8 = exp(DIR*2πi/8)
+template <int DIR, int K, typename T> __device__ __forceinline__ T tw8(T v) {
+  using R = decltype(v.x);
+  const R c = R(0.70710678118654752440084436210485);
+  if constexpr (K == 0) return v;
+  else if constexpr (K == 1) return cmul(v, T{c, DIR * c});
+  else if constexpr (K == 2) return mul_di<DIR>(v);
+  else return cmul(v, T{-c, DIR * c});
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void dft8(T* x) {
+  // even/odd split: A = DFT4(x0,x2,x4,x6), B = DFT4(x1,x3,x5,x7), X[k] = A[k] + w^k B[k]
+  T a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6];
+  T b0 = x[1], b1 = x[3], b2 = x[5], b3 = x[7];
+  dft4<DIR>(a0, a1, a2, a3);
+  dft4<DIR>(b0, b1, b2, b3);
+  b1 = tw8<DIR, 1>(b1);
+  b2 = tw8<DIR, 2>(b2);
+  b3 = tw8<DIR, 3>(b3);
+  x[0] = cadd(a0, b0); x[4] = csub(a0, b0);
+  x[1] = cadd(a1, b1); x[5] = csub(a1, b1);
+  x[2] = cadd(a2, b2); x[6] = csub(a2, b2);
+  x[3] = cadd(a3, b3); x[7] = csub(a3, b3);
+}
+
+template <int DIR, typename T> __device__ __forceinline__ T tw16(T v, int m) {
+  // exp(DIR*2πi*m/16) for the m values the radix-16 kernel uses (0..9)
+  using R = decltype(v.x);
+  const R c1 = R(0.92387953251128675612818318939679);  // cos(π/8)
+  const R s1 = R(0.38268343236508977172845998403040);  // sin(π/8)
+  const R c2 = R(0.70710678118654752440084436210485);
+  switch (m) {
+    case 0: return v;
+    case 1: return cmul(v, T{c1, DIR * s1});
+    case 2: return cmul(v, T{c2, DIR * c2});
+    case 3: return cmul(v, T{s1, DIR * c1});
+    case 4: return mul_di<DIR>(v);
+    case 6: return cmul(v, T{-c2, DIR * c2});
+    case 9: return cmul(v, T{-c1, -DIR * s1});
+    default: return v;
+  }
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void dft16(T* x) {
+  // X[k1 + 4 k2] = sum_j0 w16^{j0 k1} w4^{j0 k2} sum_j1 x[j0 + 4 j1] w4^{j1 k1}
+  T y[16];
+#pragma unroll
+  for (int j0 = 0; j0 < 4; ++j0) {
+    T a = x[j0], b = x[j0 + 4], c = x[j0 + 8], d = x[j0 + 12];
+    dft4<DIR>(a, b, c, d);
+    y[j0 * 4 + 0] = a;
+    y[j0 * 4 + 1] = tw16<DIR>(b, j0 * 1);
+    y[j0 * 4 + 2] = tw16<DIR>(c, j0 * 2);
+    y[j0 * 4 + 3] = tw16<DIR>(d, j0 * 3);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    T a = y[0 * 4 + k1], b = y[1 * 4 + k1], c = y[2 * 4 + k1], d = y[3 * 4 + k1];
+    dft4<DIR>(a, b, c, d);
+    x[k1 + 0] = a;
+    x[k1 + 4] = b;
+    x[k1 + 8] = c;
+    x[k1 + 12] = d;
+  }
+}
+
+template <int R, int DIR, typename T> __device__ __forceinline__ void dft(T* v) {
+  if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
+  else if constexpr (R == 3) dft3<DIR>(v[0], v[1], v[2]);
+  else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 8) dft8<DIR>(v);
+  else dft16<DIR>(v);
+}
+
+// ---------------------------------------------------------------- compile-time plans
+struct Plan {
+  int n, e, t, nst;
+  int r[8];
+};
+
+__host__ __device__ constexpr int plan_e(int n) {
+  if (n % 3 == 0) return n <= 12 ? n : (n <= 24 ? n : 24);
+  if (n <= 16) return n;
+  if (n <= 64 || n == 512) return 8;
+  return 16;
+}
+
+__host__ __device__ constexpr Plan make_plan(int n) {
+  Plan p{n, plan_e(n), 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  p.t = n / p.e;
+  int rem = n;
+  if (rem % 3 == 0) { p.r[p.nst++] = 3; rem /= 3; }
+  const int cand[4] = {16, 8, 4, 2};
+  while (rem > 1) {
+    for (int i = 0; i < 4; ++i) {
+      int r = cand[i];
+      if (rem % r == 0 && p.e % r == 0) { p.r[p.nst++] = r; rem /= r; break; }
+    }
+  }
+  return p;
+}
+
+template <int N> struct PlanOf {
+  static constexpr Plan P = make_plan(N);
+  static constexpr int E = P.e;
+  static constexpr int T = P.t;
+  static constexpr int NST = P.nst;
+  template <int S> static constexpr int R = P.r[S];
+  template <int S> static constexpr int NS() {
+    int ns = 1;
+    for (int i = 0; i < S; ++i) ns *= P.r[i];
+    return ns;
+  }
+};
+
+// ---------------------------------------------------------------- one Stockham transform
+// v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
+// thread's transform through addr(p). `roots[j] = exp(-2πi j/N)` (N entries).
+template <int N, int DIR, int S, typename T, typename Addr>
+__device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots) {
+  using PL = PlanOf<N>;
+  constexpr int E = PL::E, TT = PL::T;
+  constexpr int R = PL::template R<S>;
+  constexpr int NS = PL::template NS<S>();
+  constexpr int NB = E / R;             // butterflies per thread
+  constexpr bool LAST = (S == PL::NST - 1);
+  constexpr int TWS = N / (NS * R);     // root-table stride of this stage's twiddles
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + TT * b;
+    const int k = j % NS;
+    T x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T w = roots[r * k * TWS];
+        if constexpr (DIR > 0) w = cconj(w);
+        x[r] = cmul(x[r], w);
+      }
+    }
+    dft<R, DIR>(x);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + NB * r] = x[r];
+    } else {
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[addr(base + r * NS)] = x[r];
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = lds[addr(t + TT * q)];
+    __syncthreads();
+  }
+}
+
+template <int N, int DIR, int S, typename T, typename Addr>
+__device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots) {
+  if constexpr (S < PlanOf<N>::NST) {
+    fft_stage<N, DIR, S>(v, t, lds, addr, roots);
+    fft_stages<N, DIR, S + 1>(v, t, lds, addr, roots);
+  }
+}
+
+template <int N, int DIR, typename T, typename Addr>
+__device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots) {
+  fft_stages<N, DIR, 0>(v, t, lds, addr, roots);
+}
+
+}  // namespace qsf

@@ -1,0 +1,95 @@
+// qsf_launch.hpp — host-callable launchers of the HIP kernels (internal to libqsf_hip.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pass_params.hpp"
+
+namespace qsf {
+
+bool axis_size_supported(int64_t n);
+// Launches one fused axis pass; returns the number of blocks (or -1 if n is unsupported).
+int launch_axis_pass(int precision, const PassParams& p, hipStream_t s);
+int64_t axis_pass_tiles(int precision, const PassParams& p);
+
+// Per-axis phase tables of one substep, built on the device from FP64 scalars.
+//   X table: expOp(δ·(Ne(x_i)·use_ne + x_i·C)) · absorber(δ, i)
+//   P table: expOp(δ·(½p_i² + p_i·C)) · norm
+struct AxisTableArgs {
+  void* out;             // complex (precision) table of n entries
+  int64_t n;
+  int32_t kind;          // 0 = X, 1 = P
+  int32_t mode;          // QSF_MODE_IM | QSF_MODE_RE
+  double delta;
+  double C;              // coupling value for this axis (0 if the gauge acts elsewhere)
+  // X coordinates / potential / absorber
+  double xmin, dx;
+  int32_t use_ne;        // add the per-axis electron-nucleus term
+  double NE, Nsoft;
+  int32_t absorber;      // QSF_ABS_*; applied only when apply_abs
+  int32_t apply_abs;
+  int64_t nCAP;
+  double eta, tau;
+  // P coordinates (storage-axis parameters: dp, n, n/2)
+  double dp;
+  int64_t pn, pn2;
+  double norm;
+};
+void launch_axis_tables(int precision, const AxisTableArgs* a, int count, hipStream_t s);
+
+// Static pair tables exp(-iδ ee): kind PAIR_DIFF (n entries, |i-j|) or PAIR_FULL (n*n).
+struct PairTableArgs {
+  void* out;
+  int64_t n;
+  int32_t kind;          // PAIR_DIFF | PAIR_FULL
+  int32_t model;         // QSF_POT_EBERLY | QSF_POT_ECKHARDT_SACHA
+  int32_t mode;
+  double delta, EE, Esoft, xmin, dx;
+};
+void launch_pair_table(int precision, const PairTableArgs& a, hipStream_t s);
+
+// roots[j] = exp(-2πi j/n), computed in FP64 and rounded to the precision.
+void launch_roots(int precision, void* out, int64_t n, hipStream_t s);
+
+// Sum of `count` blocks of `width` doubles into out[width] (fixed order).
+void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, hipStream_t s);
+
+// Flux through border families from gathered boundary values.
+struct FluxArgs {
+  const void* gathered;  // complex (precision)
+  int32_t precision;
+  const int32_t* pts;    // per point: [slot_c, slot_plus[d], slot_minus[d] for d < D] (1 + 2D ints)
+  const double* bvec;    // per point: nb * D border components
+  int64_t npts;
+  int32_t D, nb;
+  double inv_2dx[3];
+  double vgC[3];         // velocity-gauge correction (0 if not velocity gauge)
+  double scale;          // dV / dx0
+  double* partials;      // per block: nb sums
+  double* out;           // nb sums (after reduction), may be accumulated into
+  double* accum;         // if non-null: accum[b] += out[b] * accum_weight
+  double accum_weight;
+};
+void launch_flux(const FluxArgs& a, hipStream_t s);
+
+// Element-wise multiply of a complex (precision) array by a real FP64 table value per point.
+void launch_scale(int precision, void* psi, int64_t m, double s, hipStream_t st);
+void launch_copy_convert(int from_prec, const void* src, int to_prec, void* dst, int64_t m, hipStream_t s);
+
+}  // namespace qsf
+
+namespace qsf {
+// 2D multigrid transfer pieces (multigrid.h:454-616): gather + mask the edge slice of the
+// source into `slice` (zero elsewhere), and dst += slice.
+struct SliceArgs {
+  const void* src;
+  void* slice;
+  int64_t n0, n1;
+  int32_t dirFree;       // 0 or 1
+  int64_t nCAP;
+  const double* inv_tab; // inv_mask(node) for node in [0, nCAP]
+  const double* corr_tab;// mask_correction by the other axis' node, or nullptr (not group 1)
+};
+void launch_slice_2d(int precision, const SliceArgs& a, hipStream_t s);
+void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s);
+}  // namespace qsf

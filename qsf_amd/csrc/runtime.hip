@@ -1,0 +1,1417 @@
+// runtime.hip — host runtime of libqsf_hip.so and its C ABI (include/qsf_hip.h).
+//
+// Objects: a context (device + stream), grid sets (one grid, or the 2^D regions of a
+// multigrid), grids (device state + tables), and propagators (the SplitPropagator loop of
+// propagator.h:51-499 driving fused axis passes). All numerics run in HIP kernels; this file
+// only sequences them and builds FP64 tables on the host where bit-exactness is required.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qsf_hip.h"
+#include "physics.hpp"
+#include "qsf_launch.hpp"
+
+namespace qsf {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define QSF_HIP(x)                                                                          \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) throw Error(QSF_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define QSF_REQUIRE(cond, code, msg) \
+  do {                               \
+    if (!(cond)) throw Error(code, msg); \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) QSF_HIP(hipFree(p));
+    p = nullptr;
+    QSF_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+}  // namespace qsf
+
+struct qsf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int marks = 0;
+};
+
+namespace qsf {
+
+// ----------------------------------------------------------------------------------------
+// Grid
+// ----------------------------------------------------------------------------------------
+struct Gather {        // sparse flux points of one border list (see build_gather)
+  std::vector<qsf_border_desc> borders;
+  DBuf rowptr, pos, pts, bvec, values, partials, out;
+  int64_t npts = 0, nvals = 0;
+};
+
+struct Grid {
+  qsf_ctx* ctx;
+  qsf_grid_desc d;
+  int D, prec;
+  size_t cb;
+  int64_t n[3], n2[3], m;
+  double xmin[3], dx[3], dp[3], dV, dVm, inv_m, eta = 0;
+  int freeMask, group;
+  bool bounded[3], isMain, many;
+  std::vector<int> fftAxes;
+  int pperm[3];
+  DBuf psi, aux, partials, red, kinw, masktab, invtab, corrtab, tabs;
+  std::map<int64_t, std::unique_ptr<DBuf>> roots;
+  std::map<std::pair<int64_t, int>, std::unique_ptr<DBuf>> pairs;
+  std::unique_ptr<Gather> gather;
+  int rep = QSF_REP_X;
+  int slot = 0;
+  static constexpr int kSlots = 48;
+
+  Grid(qsf_ctx* c, const qsf_grid_desc& desc) : ctx(c), d(desc) {
+    D = d.dim;
+    QSF_REQUIRE(D >= 1 && D <= 3, QSF_ERR_ARG, "dim must be 1..3");
+    prec = d.precision;
+    QSF_REQUIRE(prec == QSF_FP64 || prec == QSF_FP32_MIXED, QSF_ERR_ARG, "bad precision");
+    cb = prec == QSF_FP64 ? 16 : 8;
+    m = 1;
+    for (int i = 0; i < 3; ++i) {
+      n[i] = i < D ? d.n[i] : 1;
+      dx[i] = i < D ? d.dx[i] : 1.0;
+      n2[i] = i < D ? n[i] / 2 : 0;
+      m *= n[i];
+    }
+    // CartesianGrid_::init (coords.h:55-93)
+    inv_m = 1.0 / m;
+    dV = 1.0;
+    for (int i = 0; i < D; ++i) {
+      dV *= dx[i];
+      xmin[i] = -0.5 * (dx[i] * (n[i] - 1));
+      dp[i] = 2.0 * phys::kPi / double(n[i]) / dx[i];
+    }
+    for (int i = D; i < 3; ++i) { xmin[i] = 0; dp[i] = 0; }
+    dVm = dV * inv_m;
+    if (d.absorber == QSF_ABS_CAP) eta = phys::ipow(3.0 / double(d.nCAP), 4);
+    freeMask = d.region_free;
+    group = 0;
+    for (int i = 0; i < D; ++i) {
+      bounded[i] = !(freeMask & (1 << i));
+      group += !bounded[i];
+    }
+    isMain = group == 0;
+    many = d.multigrid != 0;
+    for (int i = 0; i < D; ++i)
+      if (isMain || bounded[i]) fftAxes.push_back(i);
+    for (int i = 0; i < 3; ++i) pperm[i] = i;
+    if (isMain && D > 1) { pperm[0] = 1; pperm[1] = 0; }  // FFTW_MPI_TRANSPOSED_OUT P layout
+    for (int i = 0; i < D; ++i)
+      QSF_REQUIRE(axis_size_supported(n[i]), QSF_ERR_UNSUPPORTED,
+                  "axis length " + std::to_string(n[i]) + " has no compiled FFT (2^k or 3*2^k, 8..4096)");
+    if (d.potential == QSF_POT_COULOMB)
+      QSF_REQUIRE(D == 1 || group == D - 1 || group == D, QSF_ERR_UNSUPPORTED,
+                  "CoulombInteraction is supported on the device for one bounded coordinate");
+    if (d.potential == QSF_POT_EBERLY || d.potential == QSF_POT_ECKHARDT_SACHA)
+      for (int a = 0; a < D; ++a)
+        for (int b = a + 1; b < D; ++b)
+          if (bounded[a] && bounded[b])
+            QSF_REQUIRE(n[a] == n[b] && dx[a] == dx[b], QSF_ERR_UNSUPPORTED,
+                        "pair interactions need equal n and dx on both axes");
+    if (many) QSF_REQUIRE(d.absorber == QSF_ABS_CAP || d.absorber == QSF_ABS_COS18, QSF_ERR_ARG,
+                          "multigrid regions need an absorber (multigrid.h:19)");
+
+    psi.ensure(m * cb);
+    QSF_HIP(hipMemsetAsync(psi.p, 0, m * cb, ctx->stream));
+    int64_t maxn = *std::max_element(n, n + 3);
+    tabs.ensure((size_t)kSlots * 3 * maxn * cb);
+    red.ensure(64 * sizeof(double));
+    build_static_tables();
+  }
+
+  hipStream_t st() const { return ctx->stream; }
+
+  const void* root(int64_t len) {
+    auto it = roots.find(len);
+    if (it != roots.end()) return it->second->p;
+    auto b = std::make_unique<DBuf>();
+    b->ensure(len * cb);
+    launch_roots(prec, b->p, len, st());
+    const void* p = b->p;
+    roots[len] = std::move(b);
+    return p;
+  }
+
+  // CAP / cos^(1/8) mask of per-axis nodes (absorbers.h:48-59), host FP64, exact fold order.
+  double mask_nodes(int k, const int64_t* nodes) const {
+    if (d.absorber == QSF_ABS_COS18) {
+      double r = 1.0;
+      for (int i = 0; i < k; ++i)
+        r = r * (nodes[i] > 0 ? std::pow(std::cos(0.5 * phys::kPi * double(nodes[i]) / double(d.nCAP)), 0.125) : 1.0);
+      return r;
+    }
+    bool allneg = true;
+    for (int i = 0; i < k; ++i) allneg = allneg && nodes[i] < 0;
+    if (allneg) return 1.0;
+    double S = 0.0;
+    for (int i = k - 1; i >= 0; --i) S = (nodes[i] > 0 ? double(phys::ipow<int64_t>(nodes[i], 4)) : 0.0) + S;
+    return std::exp(-double(S) * eta);
+  }
+
+  void build_static_tables() {
+    // <½p²> weights per axis with the reference's P coordinates (coords.h:132-138; storage axis
+    // parameters under the transposed layout)
+    int64_t maxn = *std::max_element(n, n + 3);
+    kinw.ensure(3 * maxn * sizeof(double));
+    std::vector<double> kw(3 * maxn, 0.0);
+    for (int a = 0; a < D; ++a) {
+      const int s = pperm[a];
+      for (int64_t i = 0; i < n[a]; ++i) {
+        const double p = dp[s] * (double)(i < n2[s] ? i : i - n[s]);
+        kw[a * maxn + i] = 0.5 * p * p;
+      }
+    }
+    QSF_HIP(hipMemcpy(kinw.p, kw.data(), kw.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (d.absorber != QSF_ABS_NONE && d.nCAP > 0) {
+      // region mask over clamped nodes of the bounded axes (grid.h:582-597)
+      const int64_t c = d.nCAP + 1;
+      std::vector<double> mt((size_t)std::pow(c, D));
+      for (size_t k = 0; k < mt.size(); ++k) {
+        int64_t nodes[3] = {0, 0, 0};
+        size_t r = k;
+        for (int a = D - 1; a >= 0; --a) { nodes[a] = (int64_t)(r % c); r /= c; }
+        int64_t passed[3];
+        for (int a = 0; a < D; ++a) passed[a] = bounded[a] ? nodes[a] : d.nCAP - n2[a];
+        mt[k] = mask_nodes(D, passed);
+      }
+      masktab.ensure(mt.size() * sizeof(double));
+      QSF_HIP(hipMemcpy(masktab.p, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+      // single-node inv_mask and the 2D group-1 slice correction (multigrid.h:519-548)
+      std::vector<double> inv(c), corr(c);
+      for (int64_t k = 0; k < c; ++k) {
+        int64_t nd = k;
+        inv[k] = 1.0 - mask_nodes(1, &nd);
+        corr[k] = 1 - 0.5 * (1.0 * inv[k]);
+      }
+      invtab.ensure(c * sizeof(double));
+      corrtab.ensure(c * sizeof(double));
+      QSF_HIP(hipMemcpy(invtab.p, inv.data(), c * sizeof(double), hipMemcpyHostToDevice));
+      QSF_HIP(hipMemcpy(corrtab.p, corr.data(), c * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+
+  // ---- per-substep tables (ring of slots, each with D per-axis tables) ----
+  void* next_slot() {
+    int64_t maxn = *std::max_element(n, n + 3);
+    void* p = static_cast<char*>(tabs.p) + (size_t)slot * 3 * maxn * cb;
+    slot = (slot + 1) % kSlots;
+    return p;
+  }
+  void* slot_axis(void* s, int a) const {
+    int64_t maxn = *std::max_element(n, n + 3);
+    return static_cast<char*>(s) + (size_t)a * maxn * cb;
+  }
+
+  bool reduced() const { return d.potential == QSF_POT_EBERLY || d.potential == QSF_POT_ECKHARDT_SACHA; }
+  int pair_mask() const {
+    if (!reduced()) return 0;
+    int mk = 0;
+    if (D >= 2 && bounded[0] && bounded[1]) mk |= 1;
+    if (D == 3 && bounded[1] && bounded[2]) mk |= 2;
+    if (D == 3 && bounded[2] && bounded[0]) mk |= 4;
+    return mk;
+  }
+  const void* pair_table(double delta, int mode) {
+    int64_t bits;
+    std::memcpy(&bits, &delta, 8);
+    auto key = std::make_pair(bits, mode);
+    auto it = pairs.find(key);
+    if (it != pairs.end()) return it->second->p;
+    int a = 0;
+    while (a < D && !bounded[a]) ++a;
+    PairTableArgs pa{};
+    pa.n = n[a];
+    pa.kind = d.potential == QSF_POT_EBERLY ? PAIR_DIFF : PAIR_FULL;
+    pa.model = d.potential;
+    pa.mode = mode;
+    pa.delta = delta;
+    pa.EE = d.Echarge * d.Echarge;
+    pa.Esoft = d.Esoft;
+    pa.xmin = xmin[a];
+    pa.dx = dx[a];
+    auto b = std::make_unique<DBuf>();
+    b->ensure((pa.kind == PAIR_DIFF ? pa.n : pa.n * pa.n) * cb);
+    pa.out = b->p;
+    launch_pair_table(prec, pa, st());
+    const void* p = b->p;
+    pairs[key] = std::move(b);
+    return p;
+  }
+
+  // X factor of one X substep: per-axis tables built now + the static pair table.
+  XFactor x_factor(int mode, double delta, const double* C, bool coupled) {
+    void* s = next_slot();
+    AxisTableArgs args[3];
+    for (int a = 0; a < D; ++a) {
+      AxisTableArgs& t = args[a];
+      t = AxisTableArgs{};
+      t.out = slot_axis(s, a);
+      t.n = n[a];
+      t.kind = 0;
+      t.mode = mode;
+      t.delta = delta;
+      t.C = coupled ? C[a] : 0.0;
+      t.xmin = xmin[a];
+      t.dx = dx[a];
+      t.use_ne = (d.potential != QSF_POT_NONE) && bounded[a] &&
+                 (reduced() || (d.potential == QSF_POT_COULOMB));
+      t.NE = d.Ncharge * d.Echarge;
+      t.Nsoft = d.Nsoft;
+      t.absorber = d.absorber;
+      t.apply_abs = !many;
+      t.nCAP = d.nCAP;
+      t.eta = eta;
+      t.tau = d.absorber_tau;
+      t.norm = 1.0;
+    }
+    launch_axis_tables(prec, args, D, st());
+    XFactor f{};
+    for (int a = 0; a < D; ++a) f.axis[a] = args[a].out;
+    f.pair_mask = pair_mask();
+    if (f.pair_mask) {
+      f.pair = pair_table(delta, mode);
+      f.pair_kind = d.potential == QSF_POT_EBERLY ? PAIR_DIFF : PAIR_FULL;
+      int a = 0;
+      while (a < D && !bounded[a]) ++a;
+      f.pair_n = n[a];
+    }
+    return f;
+  }
+
+  // P factor of one P substep; `norm` multiplies in the inverse-FFT normalisation.
+  void p_factor(int mode, double delta, const double* C, bool coupled, bool normalise, const void** pf) {
+    void* s = next_slot();
+    AxisTableArgs args[3];
+    for (int a = 0; a < D; ++a) {
+      AxisTableArgs& t = args[a];
+      t = AxisTableArgs{};
+      const int sa = pperm[a];
+      t.out = slot_axis(s, a);
+      t.n = n[a];
+      t.kind = 1;
+      t.mode = mode;
+      t.delta = delta;
+      t.C = coupled ? C[sa] : 0.0;
+      t.dp = dp[sa];
+      t.pn = n[sa];
+      t.pn2 = n2[sa];
+      const bool fa = std::find(fftAxes.begin(), fftAxes.end(), a) != fftAxes.end();
+      t.norm = (normalise && fa) ? 1.0 / double(n[a]) : 1.0;
+    }
+    launch_axis_tables(prec, args, D, st());
+    for (int a = 0; a < 3; ++a) pf[a] = a < D ? args[a].out : nullptr;
+  }
+
+  PassParams params(int axis, const void* src, void* dst) {
+    PassParams p{};
+    p.src = src;
+    p.dst = dst;
+    for (int i = 0; i < 3; ++i) p.n[i] = n[i];
+    p.axis = axis;
+    p.inner = 1;
+    for (int i = axis + 1; i < 3; ++i) p.inner *= n[i];
+    p.outer = 1;
+    for (int i = 0; i < axis; ++i) p.outer *= n[i];
+    p.roots = root(n[axis]);
+    p.scale = 1.0;
+    p.nCAP = d.nCAP;
+    return p;
+  }
+
+  void obs_params(ObsParams& o, bool epot) {
+    o.pot_kind = d.potential == QSF_POT_COULOMB ? 1 : d.potential == QSF_POT_EBERLY ? 2
+               : d.potential == QSF_POT_ECKHARDT_SACHA ? 3 : 0;
+    o.free_mask = freeMask;
+    o.NE = d.Ncharge * d.Echarge;
+    o.EE = d.Echarge * d.Echarge;
+    o.Nsoft = d.Nsoft;
+    o.Esoft = d.Esoft;
+    for (int i = 0; i < 3; ++i) { o.xmin[i] = xmin[i]; o.dx[i] = dx[i]; }
+    o.want_epot = epot;
+  }
+
+  int run(PassParams& p) {
+    if (p.flags & F_OBS) {
+      int64_t tiles = axis_pass_tiles(prec, p);
+      partials.ensure(tiles * 3 * sizeof(double));
+      if (!p.obs.partials) p.obs.partials = partials.as<double>();
+    }
+    int k = launch_axis_pass(prec, p, st());
+    QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
+    QSF_HIP(hipGetLastError());
+    return k;
+  }
+
+  // ---- standalone members ----
+  void fourier(int to) {
+    QSF_REQUIRE(to == QSF_REP_X || to == QSF_REP_P, QSF_ERR_ARG, "fourier: rep must be X or P");
+    if (to == rep) return;
+    if (to == QSF_REP_P) {
+      for (int a : fftAxes) {
+        PassParams p = params(a, psi.p, psi.p);
+        p.flags = F_FWD;
+        run(p);
+      }
+    } else {
+      double s = 1.0;
+      if (isMain) s = inv_m;  // normalizeAfterTwoFFT grid.h:363-367
+      else {
+        double pim = 1.0;  // partial_inv_m multigrid.h:148-150
+        for (int a = 0; a < D; ++a)
+          if (bounded[a]) pim *= 1.0 / double(n[a]);
+        s = fftAxes.empty() ? 1.0 : pim;
+      }
+      bool first = true;
+      for (int a : fftAxes) {
+        PassParams p = params(a, psi.p, psi.p);
+        p.flags = F_INV1 | (first ? F_SCALE : 0u);
+        p.scale = s;
+        first = false;
+        run(p);
+      }
+    }
+    rep = to;
+  }
+
+  void evolve(int mode, int R, double delta, const double* C, int couplesIn) {
+    const double z[3] = {0, 0, 0};
+    const double* c = C ? C : z;
+    PassParams p = params(D - 1, psi.p, psi.p);
+    if (R == QSF_REP_X) {
+      p.xa = x_factor(mode, delta, c, couplesIn == QSF_REP_X);
+      p.flags = F_XA;
+    } else {
+      p_factor(mode, delta, c, couplesIn == QSF_REP_P, false, p.pf);
+      p.flags = F_P;
+    }
+    run(p);
+  }
+
+  // Partial sums [norm, epot, ekin] of the current state, reduced on the device.
+  void observe(bool epot, bool ekin, double* host3) {
+    PassParams p = params(D - 1, psi.p, psi.p);
+    p.flags = F_OBS | F_NOSTORE;
+    obs_params(p.obs, epot);
+    if (ekin) {
+      int64_t maxn = *std::max_element(n, n + 3);
+      for (int a = 0; a < D; ++a) p.obs.kin_w[a] = kinw.as<double>() + a * maxn;
+    }
+    int tiles = run(p);
+    launch_reduce_partials(partials.as<double>(), tiles, 3, red.as<double>(), st());
+    QSF_HIP(hipMemcpyAsync(host3, red.p, 3 * sizeof(double), hipMemcpyDeviceToHost, st()));
+    QSF_HIP(hipStreamSynchronize(st()));
+  }
+
+  double average(int R, int op) {
+    double r[3];
+    observe(op == 1, op == 2, r);
+    const double vol = R == QSF_REP_X ? dV : dVm;  // vol<R> coords.h:48-53
+    return r[op] * vol;
+  }
+
+  void normalize() {
+    double r[3];
+    observe(false, false, r);
+    const double res = r[0] * dV;
+    launch_scale(prec, psi.p, m, std::sqrt(1.0 / res), st());
+  }
+
+  void mask_region() {
+    QSF_REQUIRE(masktab.p, QSF_ERR_STATE, "maskRegion needs an absorber");
+    PassParams p = params(D - 1, psi.p, psi.p);
+    p.flags = F_MASK;
+    fill_mask(p);
+    run(p);
+  }
+  void fill_mask(PassParams& p) {
+    p.mask_tab = masktab.as<double>();
+    const int64_t c = d.nCAP + 1;
+    int64_t s = 1;
+    for (int a = D - 1; a >= 0; --a) {
+      p.mask_stride[a] = bounded[a] ? s : 0;
+      s *= c;
+    }
+    p.nCAP = d.nCAP;
+  }
+
+  // ---- flux: sparse gather of boundary points (grid.h:467-573, borders.h) ----
+  void build_gather(const qsf_border_desc* b, int nb) {
+    QSF_REQUIRE(nb >= 1 && nb <= 16, QSF_ERR_ARG, "1..16 flux borders");
+    if (gather && (int)gather->borders.size() == nb &&
+        std::equal(b, b + nb, gather->borders.begin(), [](const qsf_border_desc& x, const qsf_border_desc& y) {
+          return x.kind == y.kind && x.size == y.size;
+        }))
+      return;
+    auto g = std::make_unique<Gather>();
+    g->borders.assign(b, b + nb);
+    // candidate planes: a component along axis a can be non-zero only where the centred index
+    // of axis a is one of the border positions {at-1, -at}
+    std::vector<int64_t> at;
+    for (int k = 0; k < nb; ++k) {
+      if (b[k].kind == QSF_BORDER_BOX) at.push_back(b[k].size);
+      else for (long c : {phys::bord::kNS, phys::bord::kSD, phys::bord::kDT, phys::bord::kCAP}) at.push_back(c);
+    }
+    std::sort(at.begin(), at.end());
+    at.erase(std::unique(at.begin(), at.end()), at.end());
+    std::vector<int64_t> cand;
+    int64_t stride[3];
+    for (int a = 0; a < 3; ++a) {
+      stride[a] = 1;
+      for (int e = a + 1; e < 3; ++e) stride[a] *= n[e];
+    }
+    for (int a = 0; a < D; ++a)
+      for (int64_t v : at)
+        for (int64_t ci : {v - 1, -v}) {
+          const int64_t ia = ci + n2[a];
+          if (ia < 0 || ia >= n[a]) continue;
+          const int64_t o1 = a == 0 ? 1 : 0;  // enumerate the other axes
+          (void)o1;
+          for (int64_t i0 = 0; i0 < (a == 0 ? 1 : n[0]); ++i0)
+            for (int64_t i1 = 0; i1 < (a == 1 ? 1 : n[1]); ++i1)
+              for (int64_t i2 = 0; i2 < (a == 2 ? 1 : n[2]); ++i2) {
+                int64_t id[3] = {i0, i1, i2};
+                id[a] = ia;
+                cand.push_back(id[0] * stride[0] + id[1] * stride[1] + id[2] * stride[2]);
+              }
+        }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    struct P { int64_t c; bool need[3]; std::vector<double> bv; };
+    std::vector<P> pts;
+    for (int64_t c : cand) {
+      int64_t id[3] = {c / stride[0], (c / stride[1]) % n[1], c % n[2]};
+      P q;
+      q.c = c;
+      q.need[0] = q.need[1] = q.need[2] = false;
+      q.bv.assign(nb * D, 0.0);
+      bool any = false;
+      for (int k = 0; k < nb; ++k) {
+        auto v = phys::bord::border(b[k].kind, b[k].size, D, id[0] - n2[0], D > 1 ? id[1] - n2[1] : 0,
+                                    D > 2 ? id[2] - n2[2] : 0);
+        for (int a = 0; a < D; ++a) {
+          q.bv[k * D + a] = v.c[a];
+          if (v.c[a] != 0.0) { q.need[a] = true; any = true; }
+        }
+      }
+      if (any) pts.push_back(std::move(q));
+    }
+    // needed values: the point and its ±stride neighbours (strict linear rule grid.h:470-472)
+    std::vector<int64_t> need;
+    for (auto& q : pts) {
+      need.push_back(q.c);
+      for (int a = 0; a < D; ++a)
+        if (q.need[a] && q.c - stride[a] > 0 && q.c + stride[a] < m) {
+          need.push_back(q.c + stride[a]);
+          need.push_back(q.c - stride[a]);
+        }
+    }
+    std::sort(need.begin(), need.end());
+    need.erase(std::unique(need.begin(), need.end()), need.end());
+    auto slot_of = [&](int64_t c) { return (int32_t)(std::lower_bound(need.begin(), need.end(), c) - need.begin()); };
+    const int64_t rowlen = n[D - 1], rows = m / rowlen;
+    std::vector<int32_t> rowptr(rows + 1, 0), pos(need.size());
+    for (size_t k = 0; k < need.size(); ++k) {
+      rowptr[need[k] / rowlen + 1]++;
+      pos[k] = (int32_t)(need[k] % rowlen);
+    }
+    for (int64_t r = 0; r < rows; ++r) rowptr[r + 1] += rowptr[r];
+    std::vector<int32_t> pt(pts.size() * (1 + 2 * D));
+    std::vector<double> bv(pts.size() * nb * D);
+    for (size_t k = 0; k < pts.size(); ++k) {
+      int32_t* q = &pt[k * (1 + 2 * D)];
+      q[0] = slot_of(pts[k].c);
+      for (int a = 0; a < D; ++a) {
+        const bool ok = pts[k].need[a] && pts[k].c - stride[a] > 0 && pts[k].c + stride[a] < m;
+        q[1 + a] = ok ? slot_of(pts[k].c + stride[a]) : -1;
+        q[1 + D + a] = ok ? slot_of(pts[k].c - stride[a]) : -1;
+      }
+      std::copy(pts[k].bv.begin(), pts[k].bv.end(), bv.begin() + k * nb * D);
+    }
+    g->npts = (int64_t)pts.size();
+    g->nvals = (int64_t)need.size();
+    auto up = [&](DBuf& dst, const void* src, size_t bytes) {
+      dst.ensure(std::max<size_t>(bytes, 16));
+      if (bytes) QSF_HIP(hipMemcpy(dst.p, src, bytes, hipMemcpyHostToDevice));
+    };
+    up(g->rowptr, rowptr.data(), rowptr.size() * 4);
+    up(g->pos, pos.data(), pos.size() * 4);
+    up(g->pts, pt.data(), pt.size() * 4);
+    up(g->bvec, bv.data(), bv.size() * 8);
+    g->values.ensure(std::max<int64_t>(g->nvals, 1) * cb);
+    g->partials.ensure(64 * 16 * sizeof(double));
+    g->out.ensure(16 * sizeof(double));
+    gather = std::move(g);
+  }
+
+  void set_gather(ObsParams& o) {
+    o.gather_rowptr = gather->rowptr.as<int32_t>();
+    o.gather_pos = gather->pos.as<int32_t>();
+    o.gather_out = gather->values.p;
+  }
+
+  void flux_from_gather(const double* vgC, double* dev_out, double* accum, double w) {
+    FluxArgs f{};
+    f.gathered = gather->values.p;
+    f.precision = prec;
+    f.pts = gather->pts.as<int32_t>();
+    f.bvec = gather->bvec.as<double>();
+    f.npts = gather->npts;
+    f.D = D;
+    f.nb = (int)gather->borders.size();
+    for (int a = 0; a < 3; ++a) {
+      f.inv_2dx[a] = a < D ? (1.0 / dx[a]) / 2.0 : 0.0;  // coords.h:77-78
+      f.vgC[a] = vgC ? vgC[a] : 0.0;
+    }
+    f.scale = dV / dx[0];
+    f.partials = gather->partials.as<double>();
+    f.out = dev_out ? dev_out : gather->out.as<double>();
+    f.accum = accum;
+    f.accum_weight = w;
+    launch_flux(f, st());
+  }
+
+  void flux(const double* vgC, const qsf_border_desc* b, int nb, double* out) {
+    QSF_REQUIRE(rep == QSF_REP_X, QSF_ERR_STATE, "flux needs the X representation");
+    build_gather(b, nb);
+    PassParams p = params(D - 1, psi.p, psi.p);
+    p.flags = F_OBS | F_NOSTORE;
+    obs_params(p.obs, false);
+    set_gather(p.obs);
+    run(p);
+    flux_from_gather(vgC, nullptr, nullptr, 0.0);
+    QSF_HIP(hipMemcpyAsync(out, gather->out.p, nb * sizeof(double), hipMemcpyDeviceToHost, st()));
+    QSF_HIP(hipStreamSynchronize(st()));
+  }
+
+  // ---- host transfer: FP64 interleaved in the reference layout ----
+  void upload(const double* h, int64_t count, int r) {
+    QSF_REQUIRE(count == m, QSF_ERR_ARG, "upload count must equal the grid size");
+    std::vector<double> tmp;
+    const double* srcp = h;
+    if (r == QSF_REP_P && isMain && D > 1) {  // (n1, n0, n2) -> internal (n0, n1, n2)
+      tmp.resize(2 * m);
+      for (int64_t j = 0; j < n[1]; ++j)
+        for (int64_t i = 0; i < n[0]; ++i)
+          for (int64_t k = 0; k < n[2]; ++k) {
+            const int64_t a = ((j * n[0] + i) * n[2] + k), b2 = ((i * n[1] + j) * n[2] + k);
+            tmp[2 * b2] = h[2 * a];
+            tmp[2 * b2 + 1] = h[2 * a + 1];
+          }
+      srcp = tmp.data();
+    }
+    if (prec == QSF_FP64) {
+      QSF_HIP(hipMemcpyAsync(psi.p, srcp, m * 16, hipMemcpyHostToDevice, st()));
+    } else {
+      aux.ensure(m * 16);
+      QSF_HIP(hipMemcpyAsync(aux.p, srcp, m * 16, hipMemcpyHostToDevice, st()));
+      launch_copy_convert(QSF_FP64, aux.p, QSF_FP32_MIXED, psi.p, m, st());
+    }
+    QSF_HIP(hipStreamSynchronize(st()));
+    rep = r;
+  }
+
+  void download(double* h, int64_t count) {
+    QSF_REQUIRE(count == m, QSF_ERR_ARG, "download count must equal the grid size");
+    std::vector<double> tmp;
+    double* dstp = (rep == QSF_REP_P && isMain && D > 1) ? (tmp.resize(2 * m), tmp.data()) : h;
+    if (prec == QSF_FP64) {
+      QSF_HIP(hipMemcpyAsync(dstp, psi.p, m * 16, hipMemcpyDeviceToHost, st()));
+    } else {
+      aux.ensure(m * 16);
+      launch_copy_convert(QSF_FP32_MIXED, psi.p, QSF_FP64, aux.p, m, st());
+      QSF_HIP(hipMemcpyAsync(dstp, aux.p, m * 16, hipMemcpyDeviceToHost, st()));
+    }
+    QSF_HIP(hipStreamSynchronize(st()));
+    if (dstp != h)
+      for (int64_t i = 0; i < n[0]; ++i)
+        for (int64_t j = 0; j < n[1]; ++j)
+          for (int64_t k = 0; k < n[2]; ++k) {
+            const int64_t a = ((i * n[1] + j) * n[2] + k), b2 = ((j * n[0] + i) * n[2] + k);
+            h[2 * b2] = tmp[2 * a];
+            h[2 * b2 + 1] = tmp[2 * a + 1];
+          }
+  }
+};
+
+// ----------------------------------------------------------------------------------------
+// Grid sets and the 2D multigrid transfer (multigrid.h:617-657, mpi_logic.h:103-263)
+// ----------------------------------------------------------------------------------------
+struct Source { int region; int fc; };
+
+struct Set {
+  qsf_ctx* ctx;
+  std::vector<std::unique_ptr<Grid>> g;
+  std::vector<std::vector<Source>> sources;
+  int D, groupCount = 1;
+  bool multi;
+  DBuf slice;
+
+  Set(qsf_ctx* c, const qsf_grid_desc& d) : ctx(c) {
+    D = d.dim;
+    multi = d.multigrid != 0;
+    if (!multi) {
+      g.emplace_back(new Grid(c, d));
+      return;
+    }
+    QSF_REQUIRE(D == 2, QSF_ERR_UNSUPPORTED, "device multigrid is implemented for 2D grids");
+    const int free2[4] = {QSF_AXIS_NO, QSF_AXIS_X, QSF_AXIS_Y, QSF_AXIS_XY};  // mpi_logic.h:256-259
+    for (int fa : free2) {
+      qsf_grid_desc dd = d;
+      dd.region_free = fa;
+      g.emplace_back(new Grid(c, dd));
+    }
+    groupCount = 3;
+    sources.resize(4);
+    for (int r = 0; r < 4; ++r)
+      for (int i = 0; i < 4; ++i)
+        if (__builtin_popcount(free2[r]) - __builtin_popcount(free2[i]) == 1) {
+          const int x = free2[r] ^ free2[i];
+          if (x == QSF_AXIS_X || x == QSF_AXIS_Y) sources[r].push_back({i, x});
+        }
+  }
+
+  // transfer() in group order; every source must be in X along the freed axis.
+  void transfer() {
+    for (int gI = 1; gI < groupCount; ++gI)
+      for (int sI = 0; sI < gI; ++sI)
+        for (size_t r = 0; r < g.size(); ++r) {
+          Grid& rg = *g[r];
+          if (rg.group != gI) continue;
+          const Source& s = sources[r][sI];
+          Grid& src = *g[s.region];
+          slice.ensure(rg.m * rg.cb);
+          SliceArgs a{};
+          a.src = src.psi.p;
+          a.slice = slice.p;
+          a.n0 = rg.n[0];
+          a.n1 = rg.n[1];
+          a.dirFree = s.fc == QSF_AXIS_X ? 0 : 1;
+          a.nCAP = rg.d.nCAP;
+          a.inv_tab = rg.invtab.as<double>();
+          a.corr_tab = rg.group == 1 ? rg.corrtab.as<double>() : nullptr;
+          launch_slice_2d(rg.prec, a, ctx->stream);
+          PassParams p = rg.params(a.dirFree, slice.p, slice.p);
+          p.flags = F_FWD;  // transf_slice: FFTW_FORWARD, unnormalised
+          rg.run(p);
+          launch_add(rg.prec, rg.psi.p, slice.p, rg.m, ctx->stream);
+        }
+  }
+  void post_compute() {
+    if (!multi) return;
+    transfer();
+    for (auto& gr : g)
+      if (gr->group != groupCount - 1) gr->mask_region();
+  }
+};
+
+// ----------------------------------------------------------------------------------------
+// Propagator
+// ----------------------------------------------------------------------------------------
+struct SubOp {
+  int rep;
+  double delta;
+  double C[3];
+};
+
+struct Chain {
+  std::vector<int> reps;
+  std::vector<double> mults;
+};
+
+// MultiProductSplit<Split3Base<X,P,X>,order>::Chain<c> (splitting.h:18-27, 43-58, 63-87)
+static Chain chain_of(int c) {
+  Chain ch;
+  const double base[3] = {0.5, 1.0, 0.5};
+  ch.reps.push_back(QSF_REP_X);
+  for (int k = 0; k < c; ++k) { ch.reps.push_back(QSF_REP_P); ch.reps.push_back(QSF_REP_X); }
+  ch.mults.assign(ch.reps.size(), 0.0);
+  int i = 0;
+  for (int k = 0; k < c; ++k) {
+    const double v = 1.0 / c;
+    for (int j = 0; j < 3; ++j) ch.mults[i + j] += base[j] * v;
+    i += 2;
+  }
+  return ch;
+}
+
+struct PendingX {
+  bool on = false;
+  XFactor f{};
+};
+
+struct Prop {
+  Set* set;
+  qsf_prop_desc pd;
+  phys::Coupling coup;
+  std::vector<Chain> chains;
+  int64_t max_steps = 0, step = 0;
+  double dt, timer = 0, timer_copy = 0, state_accuracy = 0;
+  double dif_energy = 1e300, last_change = 0;
+  int exec = 0;
+  int width;
+  // host rows; rows produced by fused passes are completed from the device on sync
+  std::vector<double> rows;
+  struct DevRow { int row; bool ekin; };
+  std::vector<DevRow> pending_rows;
+  DBuf devrec, fluxint;
+  int64_t devrec_cap = 0, devrec_used = 0;
+  // fused pipeline state of each grid
+  std::vector<bool> deferred;     // state is "after a P pass": pending inverse (main) + pending X
+  std::vector<PendingX> pend;
+  bool pend_obs = false;
+  double pend_obs_C[3] = {0, 0, 0};
+  int pend_row = -1;
+  bool pend_ekin = false;
+
+  Prop(Set* s, const qsf_prop_desc& p, const qsf_field_desc* f, int nf) : set(s), pd(p), coup(p.gauge, f, nf) {
+    QSF_REQUIRE(p.order >= 1 && p.order <= 8, QSF_ERR_ARG, "order must be 1..8");
+    QSF_REQUIRE(p.nborders >= 0 && p.nborders <= 16, QSF_ERR_ARG, "0..16 borders");
+    dt = p.dt;
+    for (int c = 1; c <= p.order; ++c) chains.push_back(chain_of(c));
+    if (p.mode == QSF_MODE_RE) {
+      max_steps = (int64_t)(coup.max_pulse_duration() / dt + 1);  // propagator.h:153
+      state_accuracy = 0;
+    } else {
+      max_steps = p.max_steps;
+      state_accuracy = p.state_accuracy;
+    }
+    width = 7 + p.nborders;
+    deferred.assign(s->g.size(), false);
+    pend.resize(s->g.size());
+    fluxint.ensure(16 * sizeof(double));
+    QSF_HIP(hipMemsetAsync(fluxint.p, 0, 16 * sizeof(double), s->ctx->stream));
+    // observables that change the state or steer the loop every step run unfused
+    if (p.mode == QSF_MODE_IM || (p.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF))) exec = 1;
+    if (p.nborders > 0) main().build_gather(p.borders, p.nborders);
+  }
+
+  Grid& main() { return *set->g[0]; }
+  hipStream_t st() { return set->ctx->stream; }
+  int couples() const { return coup.rep(); }
+
+  // ---- host side of makeStep: chain bookkeeping, precalc side effects, substep list ----
+  std::vector<SubOp> schedule() {
+    std::vector<SubOp> ops;
+    const size_t K = chains.size();
+    if (K > 1) timer_copy = timer;  // chainBackup (propagator.h:396-409)
+    for (size_t c = 1; c <= K; ++c) {
+      if (c > 1) timer = timer_copy;  // chainRestore (:371-380)
+      const Chain& ch = chains[c - 1];
+      for (size_t SI = 0; SI < ch.reps.size(); ++SI) {
+        const int rep = ch.reps[SI];
+        if (couples() == rep) coup.precalc(timer);  // chainEvolve (:311-332)
+        if (c == K) {
+          // only the last chain's result survives chainComplete (:381-395)
+          SubOp o{rep, dt * ch.mults[SI], {0, 0, 0}};
+          if (couples() == rep) coup.values(o.C);
+          ops.push_back(o);
+        }
+        if (couples() == rep) timer += dt * (ch.mults[SI] * 1.0);
+      }
+    }
+    ++step;
+    return ops;
+  }
+
+  // ---- unfused execution: one standalone call per reference member ----
+  void exec_simple(const std::vector<SubOp>& ops) {
+    for (size_t SI = 0; SI < ops.size(); ++SI) {
+      for (auto& gr : set->g) {
+        if (SI > 0) gr->fourier(ops[SI].rep);
+        gr->evolve(pd.mode, ops[SI].rep, ops[SI].delta, ops[SI].C, couples());
+      }
+    }
+  }
+
+  // ---- fused execution ----
+  // Each P substep is one row pass on the last axis (inverse of the previous P, the X factors,
+  // forward FFT) and column passes for the other axes; the trailing X substep of the step
+  // stays pending and is applied by the next pass that reads the grid.
+  void exec_fused(const std::vector<SubOp>& ops) {
+    for (size_t r = 0; r < set->g.size(); ++r) {
+      Grid& g = *set->g[r];
+      const bool obs_here = (r == 0) && pend_obs;
+      for (size_t SI = 0; SI < ops.size(); ++SI) {
+        const SubOp& o = ops[SI];
+        if (o.rep != QSF_REP_P) continue;
+        const SubOp& xb = ops[SI - 1];
+        XFactor fb = g.x_factor(pd.mode, xb.delta, xb.C, couples() == QSF_REP_X);
+        const void* pf[3];
+        g.p_factor(pd.mode, o.delta, o.C, couples() == QSF_REP_P, true, pf);
+        const bool first = (SI == 1);
+        run_p_substep(g, r, fb, pf, first && obs_here);
+      }
+      const SubOp& xl = ops.back();
+      pend[r].on = true;
+      pend[r].f = g.x_factor(pd.mode, xl.delta, xl.C, couples() == QSF_REP_X);
+      deferred[r] = true;
+    }
+    if (pend_obs) pend_obs = false;
+  }
+
+  void obs_into(Grid& g, PassParams& p) {
+    p.flags |= F_OBS;
+    g.obs_params(p.obs, (pd.outputs & QSF_OUT_EPOT) != 0);
+    if (pd.nborders > 0 && (pd.outputs & QSF_OUT_FLUX)) g.set_gather(p.obs);
+    if (pend_ekin) {
+      g.aux.ensure(g.m * g.cb);
+      p.obs.copy_out = g.aux.p;
+    }
+  }
+
+  void after_obs(Grid& g, int tiles) {
+    // reduce [norm, epot] of the boundary state into this row's device record
+    double* rec = devrec.as<double>() + (size_t)pend_row * (3 + 16);
+    launch_reduce_partials(g.partials.as<double>(), tiles, 3, rec, st());
+    if (pd.nborders > 0 && (pd.outputs & QSF_OUT_FLUX))
+      g.flux_from_gather(couples() == QSF_REP_P ? pend_obs_C : nullptr, rec + 3, fluxint.as<double>(), dt);
+    if (pend_ekin) {
+      // kinetic energy: forward transform of the copied boundary state, then <½p²>
+      Grid& gg = g;
+      for (int a : gg.fftAxes) {
+        PassParams q = gg.params(a, gg.aux.p, gg.aux.p);
+        q.flags = F_FWD;
+        gg.run(q);
+      }
+      PassParams q = gg.params(gg.D - 1, gg.aux.p, gg.aux.p);
+      q.flags = F_OBS | F_NOSTORE;
+      gg.obs_params(q.obs, false);
+      int64_t maxn = *std::max_element(gg.n, gg.n + 3);
+      for (int a = 0; a < gg.D; ++a) q.obs.kin_w[a] = gg.kinw.as<double>() + a * maxn;
+      int t2 = gg.run(q);
+      DBuf& tmp = gg.red;
+      launch_reduce_partials(gg.partials.as<double>(), t2, 3, tmp.as<double>(), st());
+      QSF_HIP(hipMemcpyAsync(rec + 2, tmp.as<double>() + 2, sizeof(double), hipMemcpyDeviceToDevice, st()));
+    }
+  }
+
+  void run_p_substep(Grid& g, size_t r, const XFactor& fb, const void* const* pf, bool observe) {
+    const int D = g.D;
+    const auto& ax = g.fftAxes;
+    const bool defer = deferred[r];
+    if (ax.empty()) {  // no transformed axis: pointwise pass
+      PassParams p = g.params(D - 1, g.psi.p, g.psi.p);
+      p.flags = F_XB | F_P;
+      p.xb = fb;
+      if (defer && pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+      for (int a = 0; a < 3; ++a) p.pf[a] = pf[a];
+      g.run(p);
+      deferred[r] = true;
+      pend[r].on = false;
+      return;
+    }
+    if (ax.size() == 1) {  // one transformed axis: the whole substep is one pass
+      PassParams p = g.params(ax[0], g.psi.p, g.psi.p);
+      p.flags = F_XB | F_FWD | F_P | F_INV2;
+      p.xb = fb;
+      if (defer && pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+      for (int a = 0; a < 3; ++a) p.pf[a] = pf[a];
+      int tiles = 0;
+      if (observe) obs_into(g, p);
+      tiles = g.run(p);
+      if (observe) after_obs(g, tiles);
+      deferred[r] = true;
+      pend[r].on = false;
+      return;
+    }
+    // main grid, D >= 2: row pass on the last axis
+    {
+      PassParams p = g.params(D - 1, g.psi.p, g.psi.p);
+      p.flags = F_XB | F_FWD;
+      p.xb = fb;
+      if (defer) {
+        p.flags |= F_INV1;
+        if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+      }
+      if (observe) obs_into(g, p);
+      int tiles = g.run(p);
+      if (observe) after_obs(g, tiles);
+    }
+    for (int a = D - 2; a >= 1; --a) {  // middle axes forward
+      PassParams p = g.params(a, g.psi.p, g.psi.p);
+      p.flags = F_FWD;
+      g.run(p);
+    }
+    {
+      PassParams p = g.params(0, g.psi.p, g.psi.p);
+      p.flags = F_FWD | F_P | F_INV2;
+      for (int a = 0; a < 3; ++a) p.pf[a] = pf[a];
+      g.run(p);
+    }
+    for (int a = 1; a <= D - 2; ++a) {  // middle axes inverse
+      PassParams p = g.params(a, g.psi.p, g.psi.p);
+      p.flags = F_INV1;
+      g.run(p);
+    }
+    deferred[r] = true;
+    pend[r].on = false;
+  }
+
+  // Apply whatever the pipeline left pending so every grid holds its step-boundary state.
+  void flush() {
+    for (size_t r = 0; r < set->g.size(); ++r) {
+      if (!deferred[r]) continue;
+      Grid& g = *set->g[r];
+      const bool observe = (r == 0) && pend_obs;
+      const auto& ax = g.fftAxes;
+      PassParams p = g.params(ax.size() == 1 ? ax[0] : g.D - 1, g.psi.p, g.psi.p);
+      p.flags = 0;
+      if (ax.size() >= 2) p.flags |= F_INV1;
+      if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+      if (observe) obs_into(g, p);
+      if (p.flags) {
+        int tiles = g.run(p);
+        if (observe) after_obs(g, tiles);
+      }
+      deferred[r] = false;
+      pend[r].on = false;
+    }
+    pend_obs = false;
+  }
+
+  // ---- computeEach (propagator.h:275-300), canonical output order of qsf_types.h ----
+  void request_row(bool fused) {
+    const double nan = std::nan("");
+    std::vector<double> row(width, nan);
+    row[0] = (double)step;
+    row[1] = timer;
+    const int o = pd.outputs;
+    if (!fused) {
+      Grid& g = main();
+      if (o & QSF_OUT_NORMALIZE) g.normalize();
+      double r3[3] = {nan, nan, nan};
+      if (o & (QSF_OUT_NORM | QSF_OUT_EPOT)) g.observe((o & QSF_OUT_EPOT) != 0, false, r3);
+      if (o & QSF_OUT_NORM) row[2] = r3[0] * g.dV;
+      double epot = nan, ekin = nan;
+      if (o & QSF_OUT_EPOT) row[3] = epot = r3[1] * g.dV;
+      if ((o & QSF_OUT_FLUX) && pd.nborders > 0) {
+        double C[3];
+        coup.values(C);
+        g.flux(couples() == QSF_REP_P ? C : nullptr, pd.borders, pd.nborders, &row[7]);
+        // time integral of the flux (device copy kept in step with the fused path)
+        std::vector<double> acc(16);
+        QSF_HIP(hipMemcpy(acc.data(), fluxint.p, 16 * sizeof(double), hipMemcpyDeviceToHost));
+        for (int b = 0; b < pd.nborders; ++b) acc[b] += row[7 + b] * dt;
+        QSF_HIP(hipMemcpy(fluxint.p, acc.data(), 16 * sizeof(double), hipMemcpyHostToDevice));
+      }
+      if (o & QSF_OUT_EKIN) {
+        g.fourier(QSF_REP_P);
+        double r[3];
+        g.observe(false, true, r);
+        row[4] = ekin = r[2] * g.dVm;
+        g.fourier(QSF_REP_X);
+      }
+      if (o & QSF_OUT_ETOT) row[5] = epot + ekin;
+      if (o & QSF_OUT_EDIFF) {
+        const double curr = epot + ekin;
+        row[6] = curr - last_change;
+        dif_energy = curr - last_change;
+        last_change = curr;
+      }
+      rows.insert(rows.end(), row.begin(), row.end());
+      return;
+    }
+    // fused: the values come from the next pass over the main grid
+    const int ri = (int)(rows.size() / width);
+    rows.insert(rows.end(), row.begin(), row.end());
+    if (devrec_used >= devrec_cap) {
+      sync_rows();
+      devrec_used = 0;
+    }
+    if (!devrec.p) {
+      devrec_cap = 4096;
+      devrec.ensure(devrec_cap * (3 + 16) * sizeof(double));
+    }
+    pend_obs = true;
+    pend_row = (int)devrec_used++;
+    pend_ekin = (o & (QSF_OUT_EKIN | QSF_OUT_ETOT)) != 0;
+    coup.values(pend_obs_C);
+    pending_rows.push_back({ri, pend_ekin});
+  }
+
+  void sync_rows() {
+    if (pending_rows.empty()) return;
+    if (pend_obs) flush_obs_only();
+    std::vector<double> dev((size_t)devrec_used * (3 + 16));
+    QSF_HIP(hipMemcpyAsync(dev.data(), devrec.p, dev.size() * sizeof(double), hipMemcpyDeviceToHost, st()));
+    QSF_HIP(hipStreamSynchronize(st()));
+    Grid& g = main();
+    const int o = pd.outputs;
+    for (size_t k = 0; k < pending_rows.size(); ++k) {
+      double* row = &rows[(size_t)pending_rows[k].row * width];
+      const double* d = &dev[k * (3 + 16)];
+      if (o & QSF_OUT_NORM) row[2] = d[0] * g.dV;
+      if (o & QSF_OUT_EPOT) row[3] = d[1] * g.dV;
+      if (o & QSF_OUT_EKIN) row[4] = d[2] * g.dVm;
+      if (o & QSF_OUT_ETOT) row[5] = d[1] * g.dV + d[2] * g.dVm;
+      if (o & QSF_OUT_FLUX)
+        for (int b = 0; b < pd.nborders; ++b) row[7 + b] = d[3 + b];
+    }
+    pending_rows.clear();
+    devrec_used = 0;
+  }
+
+  void flush_obs_only() { flush(); }
+
+  bool fused() const { return exec == 0; }
+
+  void compute_each() {
+    if (pd.outputs == 0) return;
+    if (fused() && step > 0) request_row(true);
+    else {
+      if (fused()) flush();
+      request_row(false);
+    }
+  }
+
+  void make_step() {
+    auto ops = schedule();
+    if (fused()) exec_fused(ops);
+    else exec_simple(ops);
+  }
+
+  void post_compute() {
+    if (!set->multi) return;
+    if (fused()) flush();
+    set->post_compute();
+  }
+
+  bool should_continue() const {
+    if (pd.mode == QSF_MODE_RE) return step <= max_steps;
+    return std::fabs(dif_energy) > state_accuracy && step != max_steps;
+  }
+  void start() { compute_each(); }
+  void loop_body() {
+    make_step();
+    compute_each();
+    post_compute();
+  }
+};
+
+}  // namespace qsf
+
+using namespace qsf;
+
+struct qsf_set { Set s; qsf_set(qsf_ctx* c, const qsf_grid_desc& d) : s(c, d) {} };
+struct qsf_grid { Grid* g; };
+struct qsf_prop { Prop p; qsf_prop(Set* s, const qsf_prop_desc& d, const qsf_field_desc* f, int nf) : p(s, d, f, nf) {} };
+
+namespace {
+thread_local std::string g_noctx_err;
+
+template <typename F> int guard(qsf_ctx* ctx, F&& f) {
+  try {
+    f();
+    return QSF_OK;
+  } catch (const Error& e) {
+    (ctx ? ctx->err : g_noctx_err) = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    (ctx ? ctx->err : g_noctx_err) = e.what();
+    return QSF_ERR_ARG;
+  }
+}
+std::map<const qsf_grid*, std::unique_ptr<qsf_grid>> g_handles;
+qsf_grid* handle_of(Grid* g) {
+  for (auto& kv : g_handles)
+    if (kv.second->g == g) return kv.second.get();
+  auto h = std::make_unique<qsf_grid>();
+  h->g = g;
+  qsf_grid* p = h.get();
+  g_handles[p] = std::move(h);
+  return p;
+}
+}  // namespace
+
+extern "C" {
+
+int qsf_ctx_create(int device, qsf_ctx** out) {
+  if (!out) return QSF_ERR_ARG;
+  auto* c = new qsf_ctx();
+  c->device = device;
+  int rc = guard(nullptr, [&] {
+    int count = 0;
+    QSF_HIP(hipGetDeviceCount(&count));
+    QSF_REQUIRE(device >= 0 && device < count, QSF_ERR_ARG, "no such device");
+    QSF_HIP(hipSetDevice(device));
+    QSF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    QSF_HIP(hipEventCreate(&c->ev[0]));
+    QSF_HIP(hipEventCreate(&c->ev[1]));
+  });
+  if (rc != QSF_OK) {
+    delete c;
+    *out = nullptr;
+    return rc;
+  }
+  *out = c;
+  return QSF_OK;
+}
+
+int qsf_ctx_destroy(qsf_ctx* c) {
+  if (!c) return QSF_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return QSF_OK;
+}
+
+const char* qsf_last_error(const qsf_ctx* c) { return c ? c->err.c_str() : g_noctx_err.c_str(); }
+
+int qsf_ctx_synchronize(qsf_ctx* c) {
+  if (!c) return QSF_ERR_ARG;
+  return guard(c, [&] { QSF_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+int qsf_timer_mark(qsf_ctx* c) {
+  if (!c) return QSF_ERR_ARG;
+  return guard(c, [&] {
+    QSF_HIP(hipEventRecord(c->ev[c->marks & 1], c->stream));
+    ++c->marks;
+  });
+}
+
+int qsf_timer_elapsed_ms(qsf_ctx* c, double* ms) {
+  if (!c || !ms) return QSF_ERR_ARG;
+  return guard(c, [&] {
+    QSF_REQUIRE(c->marks >= 2, QSF_ERR_STATE, "need two marks");
+    hipEvent_t a = c->ev[(c->marks - 2) & 1], b = c->ev[(c->marks - 1) & 1];
+    QSF_HIP(hipEventSynchronize(b));
+    float f = 0;
+    QSF_HIP(hipEventElapsedTime(&f, a, b));
+    *ms = f;
+  });
+}
+
+int qsf_set_create(qsf_ctx* c, const qsf_grid_desc* d, qsf_set** out) {
+  if (!c || !d || !out) return QSF_ERR_ARG;
+  *out = nullptr;
+  return guard(c, [&] {
+    QSF_HIP(hipSetDevice(c->device));
+    *out = new qsf_set(c, *d);
+  });
+}
+
+int qsf_set_destroy(qsf_set* s) {
+  if (!s) return QSF_ERR_ARG;
+  (void)hipStreamSynchronize(s->s.ctx->stream);
+  for (auto& g : s->s.g) {
+    for (auto it = g_handles.begin(); it != g_handles.end();)
+      if (it->second->g == g.get()) it = g_handles.erase(it);
+      else ++it;
+  }
+  delete s;
+  return QSF_OK;
+}
+
+int qsf_set_count(const qsf_set* s, int* n) {
+  if (!s || !n) return QSF_ERR_ARG;
+  *n = (int)s->s.g.size();
+  return QSF_OK;
+}
+
+int qsf_set_grid(qsf_set* s, int r, qsf_grid** out) {
+  if (!s || !out || r < 0 || r >= (int)s->s.g.size()) return QSF_ERR_ARG;
+  *out = handle_of(s->s.g[r].get());
+  return QSF_OK;
+}
+
+int qsf_set_post_compute(qsf_set* s) {
+  if (!s) return QSF_ERR_ARG;
+  return guard(s->s.ctx, [&] { s->s.post_compute(); });
+}
+
+int qsf_grid_size(const qsf_grid* g, int64_t* m) {
+  if (!g || !m) return QSF_ERR_ARG;
+  *m = g->g->m;
+  return QSF_OK;
+}
+int qsf_grid_rep(const qsf_grid* g, int* r) {
+  if (!g || !r) return QSF_ERR_ARG;
+  *r = g->g->rep;
+  return QSF_OK;
+}
+int qsf_grid_upload(qsf_grid* g, const double* h, int64_t count, int rep) {
+  if (!g || !h) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->upload(h, count, rep); });
+}
+int qsf_grid_download(qsf_grid* g, double* h, int64_t count) {
+  if (!g || !h) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->download(h, count); });
+}
+int qsf_fourier(qsf_grid* g, int to) {
+  if (!g) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->fourier(to); });
+}
+int qsf_evolve(qsf_grid* g, int mode, int rep, double delta, const double* C, int couples_in) {
+  if (!g) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] {
+    QSF_REQUIRE(rep == g->g->rep, QSF_ERR_STATE, "evolve: grid is in the other representation");
+    QSF_REQUIRE(mode == QSF_MODE_IM || mode == QSF_MODE_RE, QSF_ERR_ARG, "bad mode");
+    g->g->evolve(mode, rep, delta, C, couples_in);
+  });
+}
+int qsf_average(qsf_grid* g, int rep, int op, double* out) {
+  if (!g || !out) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] {
+    QSF_REQUIRE(rep == g->g->rep, QSF_ERR_STATE, "average: grid is in the other representation");
+    QSF_REQUIRE(op >= 0 && op <= 2, QSF_ERR_ARG, "op must be 0..2");
+    *out = g->g->average(rep, op);
+  });
+}
+int qsf_normalize(qsf_grid* g) {
+  if (!g) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->normalize(); });
+}
+int qsf_flux(qsf_grid* g, const double* vgC, const qsf_border_desc* b, int nb, double* out) {
+  if (!g || !b || !out) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->flux(vgC, b, nb, out); });
+}
+int qsf_mask_region(qsf_grid* g) {
+  if (!g) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->mask_region(); });
+}
+
+int qsf_prop_create(qsf_set* s, const qsf_prop_desc* d, const qsf_field_desc* f, int nf, qsf_prop** out) {
+  if (!s || !d || !out || (nf > 0 && !f)) return QSF_ERR_ARG;
+  *out = nullptr;
+  return guard(s->s.ctx, [&] { *out = new qsf_prop(&s->s, *d, f, nf); });
+}
+int qsf_prop_destroy(qsf_prop* p) {
+  if (!p) return QSF_ERR_ARG;
+  (void)hipStreamSynchronize(p->p.st());
+  delete p;
+  return QSF_OK;
+}
+int qsf_prop_max_steps(const qsf_prop* p, int64_t* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  *o = p->p.max_steps;
+  return QSF_OK;
+}
+int qsf_prop_start(qsf_prop* p) {
+  if (!p) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] { p->p.start(); });
+}
+int qsf_prop_should_continue(qsf_prop* p, int* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  *o = p->p.should_continue();
+  return QSF_OK;
+}
+int qsf_prop_step(qsf_prop* p) {
+  if (!p) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] { p->p.loop_body(); });
+}
+int qsf_prop_run_steps(qsf_prop* p, int64_t k, int64_t* done) {
+  if (!p) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    int64_t i = 0;
+    while (i < k && p->p.should_continue()) {
+      p->p.loop_body();
+      ++i;
+    }
+    if (done) *done = i;
+  });
+}
+int qsf_prop_finish(qsf_prop* p) {
+  if (!p) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    p->p.loop_body();
+    p->p.flush();
+    p->p.sync_rows();
+  });
+}
+int qsf_prop_step_count(const qsf_prop* p, int64_t* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  *o = p->p.step;
+  return QSF_OK;
+}
+int qsf_prop_time(const qsf_prop* p, double* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  *o = p->p.timer;
+  return QSF_OK;
+}
+int qsf_prop_coupling(const qsf_prop* p, double* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  p->p.coup.values(o);
+  return QSF_OK;
+}
+int qsf_prop_record_width(const qsf_prop* p, int* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  *o = p->p.width;
+  return QSF_OK;
+}
+int qsf_prop_nrecords(qsf_prop* p, int* o) {
+  if (!p || !o) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    p->p.flush();
+    p->p.sync_rows();
+    *o = (int)(p->p.rows.size() / p->p.width);
+  });
+}
+int qsf_prop_records(qsf_prop* p, double* out, int max_rows) {
+  if (!p || !out) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    p->p.flush();
+    p->p.sync_rows();
+    const size_t n = std::min<size_t>((size_t)max_rows * p->p.width, p->p.rows.size());
+    std::memcpy(out, p->p.rows.data(), n * sizeof(double));
+  });
+}
+int qsf_prop_flux_integral(qsf_prop* p, double* out) {
+  if (!p || !out) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    p->p.flush();
+    QSF_HIP(hipMemcpyAsync(out, p->p.fluxint.p, p->p.pd.nborders * sizeof(double), hipMemcpyDeviceToHost,
+                           p->p.st()));
+    QSF_HIP(hipStreamSynchronize(p->p.st()));
+  });
+}
+int qsf_prop_set_exec(qsf_prop* p, int mode) {
+  if (!p || (mode != 0 && mode != 1)) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    QSF_REQUIRE(mode == 1 || !(p->p.pd.mode == QSF_MODE_IM || (p->p.pd.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF))),
+                QSF_ERR_UNSUPPORTED, "imaginary time, Normalize and ENERGY_DIFFERENCE run unfused");
+    p->p.flush();
+    p->p.exec = mode;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,311 @@
+"""ctypes binding of libqsf_hip.so (C ABI in include/qsf_hip.h).
+
+The library is the product: every numeric operation below runs in its HIP kernels. Loading
+fails loudly when the library is missing; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import qsf_types as T
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqsf_hip.so")
+_LIB = None
+
+# (restype, argtypes) of every exported entry point
+_P, _D, _I, _I64 = C.c_void_p, C.c_double, C.c_int, C.c_int64
+_dp = C.POINTER(C.c_double)
+SIGNATURES = {
+    "qsf_ctx_create": (_I, [_I, C.POINTER(_P)]),
+    "qsf_ctx_destroy": (_I, [_P]),
+    "qsf_last_error": (C.c_char_p, [_P]),
+    "qsf_ctx_synchronize": (_I, [_P]),
+    "qsf_timer_mark": (_I, [_P]),
+    "qsf_timer_elapsed_ms": (_I, [_P, _dp]),
+    "qsf_set_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(_P)]),
+    "qsf_set_destroy": (_I, [_P]),
+    "qsf_set_count": (_I, [_P, C.POINTER(_I)]),
+    "qsf_set_grid": (_I, [_P, _I, C.POINTER(_P)]),
+    "qsf_set_post_compute": (_I, [_P]),
+    "qsf_grid_size": (_I, [_P, C.POINTER(_I64)]),
+    "qsf_grid_rep": (_I, [_P, C.POINTER(_I)]),
+    "qsf_grid_upload": (_I, [_P, _dp, _I64, _I]),
+    "qsf_grid_download": (_I, [_P, _dp, _I64]),
+    "qsf_fourier": (_I, [_P, _I]),
+    "qsf_evolve": (_I, [_P, _I, _I, _D, _dp, _I]),
+    "qsf_average": (_I, [_P, _I, _I, _dp]),
+    "qsf_normalize": (_I, [_P]),
+    "qsf_flux": (_I, [_P, _dp, C.POINTER(T.BorderDesc), _I, _dp]),
+    "qsf_mask_region": (_I, [_P]),
+    "qsf_prop_create": (_I, [_P, C.POINTER(T.PropDesc), C.POINTER(T.FieldDesc), _I, C.POINTER(_P)]),
+    "qsf_prop_destroy": (_I, [_P]),
+    "qsf_prop_max_steps": (_I, [_P, C.POINTER(_I64)]),
+    "qsf_prop_start": (_I, [_P]),
+    "qsf_prop_should_continue": (_I, [_P, C.POINTER(_I)]),
+    "qsf_prop_step": (_I, [_P]),
+    "qsf_prop_run_steps": (_I, [_P, _I64, C.POINTER(_I64)]),
+    "qsf_prop_finish": (_I, [_P]),
+    "qsf_prop_step_count": (_I, [_P, C.POINTER(_I64)]),
+    "qsf_prop_time": (_I, [_P, _dp]),
+    "qsf_prop_coupling": (_I, [_P, _dp]),
+    "qsf_prop_record_width": (_I, [_P, C.POINTER(_I)]),
+    "qsf_prop_nrecords": (_I, [_P, C.POINTER(_I)]),
+    "qsf_prop_records": (_I, [_P, _dp, _I]),
+    "qsf_prop_flux_integral": (_I, [_P, _dp]),
+    "qsf_prop_set_exec": (_I, [_P, _I]),
+}
+
+
+def load():
+    """Load libqsf_hip.so and bind its C ABI (raises if it is missing)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libqsf_hip.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        _LIB = L
+    return _LIB
+
+
+class QSFError(RuntimeError):
+    pass
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device and one HIP stream (qsf_ctx)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = _P()
+        rc = L.qsf_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise QSFError(f"qsf_ctx_create: {L.qsf_last_error(None).decode()}")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = load().qsf_last_error(self.h).decode()
+            raise QSFError(f"{what} failed ({rc}): {msg}")
+
+    def synchronize(self):
+        self.check(load().qsf_ctx_synchronize(self.h), "synchronize")
+
+    def mark(self):
+        self.check(load().qsf_timer_mark(self.h), "timer_mark")
+
+    def elapsed_ms(self):
+        v = C.c_double()
+        self.check(load().qsf_timer_elapsed_ms(self.h, C.byref(v)), "timer_elapsed")
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qsf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grid:
+    """One device grid of a GridSet (the HamWF member surface)."""
+
+    def __init__(self, owner, handle):
+        self._owner, self.h = owner, handle
+        self.ctx = owner.ctx
+        m = C.c_int64()
+        self.ctx.check(load().qsf_grid_size(handle, C.byref(m)), "grid_size")
+        self.m = m.value
+
+    @property
+    def rep(self):
+        r = C.c_int()
+        self.ctx.check(load().qsf_grid_rep(self.h, C.byref(r)), "grid_rep")
+        return r.value
+
+    def set_psi(self, psi, rep=T.REP_X):
+        a = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128)).ravel()
+        self.ctx.check(load().qsf_grid_upload(self.h, _dptr(a), a.size, rep), "upload")
+
+    def psi(self):
+        a = np.empty(self.m, np.complex128)
+        self.ctx.check(load().qsf_grid_download(self.h, _dptr(a), self.m), "download")
+        return a
+
+    def fourier(self, rep):
+        self.ctx.check(load().qsf_fourier(self.h, rep), "fourier")
+
+    def evolve(self, mode, rep, delta, coupling=None, couples_in=None):
+        c = None
+        if coupling is not None:
+            self._c = np.ascontiguousarray(coupling, dtype=np.float64)
+            c = _dptr(self._c)
+        ci = rep if (couples_in is None and coupling is not None) else (couples_in or 0)
+        self.ctx.check(load().qsf_evolve(self.h, mode, rep, delta, c, ci), "evolve")
+
+    def average(self, rep, op):
+        v = C.c_double()
+        self.ctx.check(load().qsf_average(self.h, rep, op, C.byref(v)), "average")
+        return v.value
+
+    def normalize(self):
+        self.ctx.check(load().qsf_normalize(self.h), "normalize")
+
+    def flux(self, borders, vg_coupling=None):
+        arr = (T.BorderDesc * len(borders))()
+        for i, b in enumerate(borders):
+            kind, size = b if isinstance(b, tuple) else (b, 0)
+            arr[i].kind, arr[i].size = kind, size
+        out = np.zeros(len(borders))
+        vg = None
+        if vg_coupling is not None:
+            self._vg = np.ascontiguousarray(vg_coupling, dtype=np.float64)
+            vg = _dptr(self._vg)
+        self.ctx.check(load().qsf_flux(self.h, vg, arr, len(borders), _dptr(out)), "flux")
+        return out
+
+    def mask_region(self):
+        self.ctx.check(load().qsf_mask_region(self.h), "mask_region")
+
+
+class GridSet:
+    """One grid (multigrid=0) or the 2^D multigrid regions (multigrid=1) on the device."""
+
+    def __init__(self, ctx, desc):
+        self.ctx, self.desc = ctx, desc
+        h = _P()
+        ctx.check(load().qsf_set_create(ctx.h, C.byref(desc), C.byref(h)), "set_create")
+        self.h = h
+        n = C.c_int()
+        ctx.check(load().qsf_set_count(h, C.byref(n)), "set_count")
+        self.grids = []
+        for r in range(n.value):
+            g = _P()
+            ctx.check(load().qsf_set_grid(h, r, C.byref(g)), "set_grid")
+            self.grids.append(Grid(self, g))
+
+    def __getitem__(self, r):
+        return self.grids[r]
+
+    def __len__(self):
+        return len(self.grids)
+
+    def post_compute(self):
+        self.ctx.check(load().qsf_set_post_compute(self.h), "post_compute")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qsf_set_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Propagator:
+    """SplitPropagator over a device GridSet (fused axis passes by default)."""
+
+    def __init__(self, gset, pdesc, fields, exec_mode=0):
+        self.gset, self.ctx = gset, gset.ctx
+        self._fields = T.field_array(fields)
+        self._pd = pdesc
+        h = _P()
+        self.ctx.check(load().qsf_prop_create(gset.h, C.byref(pdesc), self._fields, len(fields),
+                                              C.byref(h)), "prop_create")
+        self.h = h
+        if exec_mode:
+            self.ctx.check(load().qsf_prop_set_exec(h, exec_mode), "prop_set_exec")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qsf_prop_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _i64(self, fn):
+        v = C.c_int64()
+        self.ctx.check(fn(self.h, C.byref(v)), fn.__name__)
+        return v.value
+
+    @property
+    def max_steps(self):
+        return self._i64(load().qsf_prop_max_steps)
+
+    @property
+    def step_count(self):
+        return self._i64(load().qsf_prop_step_count)
+
+    @property
+    def time(self):
+        v = C.c_double()
+        self.ctx.check(load().qsf_prop_time(self.h, C.byref(v)), "prop_time")
+        return v.value
+
+    def coupling(self):
+        out = np.zeros(3)
+        self.ctx.check(load().qsf_prop_coupling(self.h, _dptr(out)), "prop_coupling")
+        return out
+
+    def start(self):
+        self.ctx.check(load().qsf_prop_start(self.h), "prop_start")
+
+    def should_continue(self):
+        v = C.c_int()
+        self.ctx.check(load().qsf_prop_should_continue(self.h, C.byref(v)), "should_continue")
+        return bool(v.value)
+
+    def step(self):
+        self.ctx.check(load().qsf_prop_step(self.h), "prop_step")
+
+    def run_steps(self, k):
+        done = C.c_int64()
+        self.ctx.check(load().qsf_prop_run_steps(self.h, k, C.byref(done)), "prop_run_steps")
+        return done.value
+
+    def finish(self):
+        self.ctx.check(load().qsf_prop_finish(self.h), "prop_finish")
+
+    def records(self):
+        n, w = C.c_int(), C.c_int()
+        self.ctx.check(load().qsf_prop_nrecords(self.h, C.byref(n)), "nrecords")
+        self.ctx.check(load().qsf_prop_record_width(self.h, C.byref(w)), "record_width")
+        out = np.zeros(n.value * w.value)
+        if n.value:
+            self.ctx.check(load().qsf_prop_records(self.h, _dptr(out), n.value), "records")
+        return out.reshape(n.value, w.value)
+
+    def flux_integral(self):
+        out = np.zeros(max(1, self._pd.nborders))
+        self.ctx.check(load().qsf_prop_flux_integral(self.h, _dptr(out)), "flux_integral")
+        return out[: self._pd.nborders]
+
+    def run(self, limit=None):
+        """run() (propagator.h:420-462); `limit` caps the loop iterations for small tests."""
+        self.start()
+        k = 0
+        while self.should_continue() and (limit is None or k < limit):
+            self.step()
+            k += 1
+        if limit is None or k < limit:
+            self.finish()
+        return self.records()

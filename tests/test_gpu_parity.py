@@ -1,0 +1,307 @@
+"""Parity of the HIP path (libqsf_hip.so through its C ABI) with the CPU FP64 oracle.
+
+Tolerances: FP64 kernels L2(ψ) relative error ≤ 1e-10 (the bar the north star sets for the FP64
+path; actual errors are ~1e-14); FP32-mixed kernels ≤ 2e-5 relative per operation. Bit-exact
+operations (region masks, border tables and their indexing) are compared with array_equal.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from qsf_amd import qsf_types as T
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+FP64_TOL = 1e-10
+FP32_TOL = 2e-5
+
+
+def crand(n):
+    return RNG.normal(size=n) + 1j * RNG.normal(size=n)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from qsf_amd import hip
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def pair(ctx, desc):
+    from qsf_amd import hip
+    print(f"[case] dim={desc.dim} n={list(desc.n)[:desc.dim]} prec={desc.precision} "
+          f"pot={desc.potential} abs={desc.absorber} mg={desc.multigrid}", flush=True)
+    return hip.GridSet(ctx, desc), O.GridSet(desc)
+
+
+# ------------------------------------------------------------------------------------ FFT
+SIZES_1D = [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 12, 24, 48, 96, 192, 384, 768, 1536, 3072]
+
+
+@pytest.mark.parametrize("n", SIZES_1D)
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32_MIXED])
+def test_fft_1d(ctx, n, prec):
+    d = T.grid_desc(1, n, 0.1, precision=prec)
+    dev, ref = pair(ctx, d)
+    x = crand(n)
+    dev[0].set_psi(x)
+    ref[0].set_psi(x)
+    dev[0].fourier(T.REP_P)
+    ref[0].fourier(T.REP_P)
+    tol = FP64_TOL if prec == T.FP64 else FP32_TOL
+    assert rel(dev[0].psi(), ref[0].psi()) < tol
+    dev[0].fourier(T.REP_X)
+    assert rel(dev[0].psi(), x) < tol
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (256, 256), (2048, 16), (12, 1024)])
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32_MIXED])
+def test_fft_2d_transposed_layout(ctx, shape, prec):
+    d = T.grid_desc(2, list(shape), [0.2, 0.3], precision=prec)
+    dev, ref = pair(ctx, d)
+    x = crand(shape[0] * shape[1])
+    for g in (dev[0], ref[0]):
+        g.set_psi(x)
+        g.fourier(T.REP_P)
+    tol = FP64_TOL if prec == T.FP64 else FP32_TOL
+    assert rel(dev[0].psi(), ref[0].psi()) < tol   # both in FFTW's (n1, n0) P layout
+
+
+@pytest.mark.parametrize("shape", [(16, 32, 8), (32, 32, 32), (64, 16, 128)])
+def test_fft_3d(ctx, shape):
+    d = T.grid_desc(3, list(shape), [0.3, 0.3, 0.3])
+    dev, ref = pair(ctx, d)
+    x = crand(int(np.prod(shape)))
+    for g in (dev[0], ref[0]):
+        g.set_psi(x)
+        g.fourier(T.REP_P)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    for g in (dev[0], ref[0]):
+        g.fourier(T.REP_X)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+
+
+# ------------------------------------------------------------------------------------ evolve
+CASES = [
+    dict(dim=1, n=256, dx=0.1, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1,
+         absorber=T.ABS_CAP, nCAP=26),
+    dict(dim=1, n=512, dx=0.1, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1,
+         absorber=T.ABS_COS18, nCAP=51, absorber_tau=0.05),
+    dict(dim=2, n=64, dx=0.2, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5,
+         Esoft=0.33, absorber=T.ABS_CAP, nCAP=8),
+    dict(dim=3, n=32, dx=0.3, potential=T.POT_ECKHARDT_SACHA, Ncharge=3, Echarge=-1, Nsoft=1,
+         Esoft=1, absorber=T.ABS_CAP, nCAP=6),
+]
+
+
+def desc_of(c, prec=T.FP64, **kw):
+    c = dict(c)
+    c.update(kw)
+    dim, n, dx = c.pop("dim"), c.pop("n"), c.pop("dx")
+    return T.grid_desc(dim, n, dx, precision=prec, **c)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("mode", [T.MODE_RE, T.MODE_IM])
+def test_evolve_x_and_p(ctx, case, mode):
+    d = desc_of(CASES[case])
+    dev, ref = pair(ctx, d)
+    x = crand(dev[0].m) * 0.1
+    for g in (dev[0], ref[0]):
+        g.set_psi(x)
+    C = np.array([0.031, -0.017, 0.009])
+    for g in (dev[0], ref[0]):
+        g.evolve(mode, T.REP_X, 0.025, C, T.REP_X)   # length gauge coupling in X
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    for g in (dev[0], ref[0]):
+        g.fourier(T.REP_P)
+        g.evolve(mode, T.REP_P, 0.05, C, T.REP_P)    # velocity gauge coupling in P
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    for g in (dev[0], ref[0]):
+        g.fourier(T.REP_X)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_averages(ctx, case):
+    d = desc_of(CASES[case])
+    dev, ref = pair(ctx, d)
+    x = crand(dev[0].m) * 0.1
+    for g in (dev[0], ref[0]):
+        g.set_psi(x)
+    for op in (0, 1):
+        assert np.isclose(dev[0].average(T.REP_X, op), ref[0].average(T.REP_X, op), rtol=1e-12)
+    for g in (dev[0], ref[0]):
+        g.fourier(T.REP_P)
+    assert np.isclose(dev[0].average(T.REP_P, 2), ref[0].average(T.REP_P, 2), rtol=1e-12)
+    assert np.isclose(dev[0].average(T.REP_P, 0), ref[0].average(T.REP_P, 0), rtol=1e-12)
+    for g in (dev[0], ref[0]):
+        g.fourier(T.REP_X)
+        g.normalize()
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+
+
+# ------------------------------------------------------------------------------------ flux
+@pytest.mark.parametrize("dim,n,borders", [
+    (1, 256, [(T.BORDER_BOX, 100)]),
+    (2, 128, [(T.BORDER_BOX, 40)] + [(b, 0) for b in T.ZOA_FLUX_2D]),
+    (3, 32, [(T.BORDER_BOX, 10), (T.BORDER_N2S, 0), (T.BORDER_D2T, 0)]),
+])
+def test_flux(ctx, dim, n, borders):
+    d = T.grid_desc(dim, n, 0.5)
+    dev, ref = pair(ctx, d)
+    x = crand(dev[0].m)
+    for g in (dev[0], ref[0]):
+        g.set_psi(x)
+    A = [0.2, -0.1, 0.05]
+    for vg in (None, A):
+        a = dev[0].flux(borders, vg)
+        b = ref[0].flux(borders, vg)
+        assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.abs(b).max())
+
+
+# ------------------------------------------------------------------------------------ masks
+@pytest.mark.parametrize("absorber", [T.ABS_CAP, T.ABS_COS18])
+def test_mask_region_bit_exact(ctx, absorber):
+    d = T.grid_desc(2, 64, 0.5, absorber=absorber, nCAP=12, multigrid=1)
+    dev, ref = pair(ctx, d)
+    for r in range(4):
+        x = crand(dev[r].m)
+        dev[r].set_psi(x)
+        ref[r].set_psi(x)
+        dev[r].mask_region()
+        ref[r].mask_region()
+        assert np.array_equal(dev[r].psi(), ref[r].psi()), f"region {r}"
+
+
+def test_multigrid_post_compute(ctx):
+    d = T.grid_desc(2, 64, 0.5, absorber=T.ABS_CAP, nCAP=8, multigrid=1)
+    dev, ref = pair(ctx, d)
+    for r in range(4):
+        x = crand(dev[r].m)
+        dev[r].set_psi(x)
+        ref[r].set_psi(x)
+    dev.post_compute()
+    ref.post_compute()
+    for r in range(4):
+        assert rel(dev[r].psi(), ref[r].psi()) < FP64_TOL, f"region {r}"
+
+
+# ------------------------------------------------------------------------------------ propagation
+def run_both(ctx, desc, pdesc, fields, psi0, limit, exec_mode=0):
+    from qsf_amd import hip
+    dev = hip.GridSet(ctx, desc)
+    ref = O.GridSet(desc)
+    for r in range(len(ref)):
+        dev[r].set_psi(psi0 if r == 0 else np.zeros_like(psi0))
+        ref[r].set_psi(psi0 if r == 0 else np.zeros_like(psi0))
+    pd = hip.Propagator(dev, pdesc, fields, exec_mode=exec_mode)
+    po = O.Propagator(ref, pdesc, fields)
+    rd = pd.run(limit)
+    ro = po.run(limit)
+    return dev, ref, pd, po, rd, ro
+
+
+@pytest.mark.parametrize("exec_mode", [0, 1])
+def test_propagation_1d_length_gauge(ctx, exec_mode):
+    n, dx = 512, 0.1
+    d = T.grid_desc(1, n, dx, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1,
+                    absorber=T.ABS_COS18, nCAP=51, absorber_tau=0.05)
+    x = -0.5 * dx * (n - 1) + dx * np.arange(n)
+    psi0 = (np.exp(-x * x) + 0j) / np.sqrt(np.sqrt(np.pi / 2))
+    f = [T.field_desc(T.AXIS_X, 0.0534, 0.057, 4)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_LENGTH,
+                        outputs=T.OUT_NORM | T.OUT_EPOT | T.OUT_EKIN | T.OUT_ETOT | T.OUT_FLUX,
+                        borders=[(T.BORDER_BOX, 150)])
+    dev, ref, pd, po, rd, ro = run_both(ctx, d, pdesc, f, psi0, 20, exec_mode)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    assert rd.shape == ro.shape
+    assert np.allclose(rd[:, :2], ro[:, :2], rtol=0, atol=1e-12)   # step, time
+    assert np.allclose(rd[:, 2:6], ro[:, 2:6], rtol=1e-11, atol=1e-13)
+    assert np.allclose(rd[:, 7:], ro[:, 7:], rtol=1e-9, atol=1e-13)
+
+
+@pytest.mark.parametrize("exec_mode", [0, 1])
+def test_propagation_2d_velocity_gauge_order2(ctx, exec_mode):
+    n, dx = 128, 0.2
+    d = T.grid_desc(2, n, dx, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                    absorber=T.ABS_CAP, nCAP=16)
+    psi0 = crand(n * n) * 0.01
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    pdesc = T.prop_desc(T.MODE_RE, 2, 0.02, gauge=T.GAUGE_VELOCITY,
+                        outputs=T.OUT_NORM | T.OUT_EPOT | T.OUT_FLUX,
+                        borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+    dev, ref, pd, po, rd, ro = run_both(ctx, d, pdesc, f, psi0, 15, exec_mode)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    assert np.allclose(rd[:, 2:4], ro[:, 2:4], rtol=1e-11)
+    assert np.allclose(rd[:, 7:], ro[:, 7:], rtol=1e-8, atol=1e-12 * np.abs(ro[:, 7:]).max())
+
+
+@pytest.mark.parametrize("exec_mode", [0, 1])
+def test_propagation_3d_length_gauge(ctx, exec_mode):
+    n, dx = 32, 0.3
+    d = T.grid_desc(3, n, dx, potential=T.POT_ECKHARDT_SACHA, Ncharge=3, Echarge=-1, Nsoft=1, Esoft=1,
+                    absorber=T.ABS_CAP, nCAP=6)
+    psi0 = crand(n ** 3) * 0.01
+    f = [T.field_desc(T.AXIS_XYZ, 0.1, 0.057, 2)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_LENGTH,
+                        outputs=T.OUT_NORM | T.OUT_EPOT | T.OUT_EKIN | T.OUT_FLUX,
+                        borders=[(T.BORDER_BOX, 10)])
+    dev, ref, pd, po, rd, ro = run_both(ctx, d, pdesc, f, psi0, 10, exec_mode)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    assert np.allclose(rd[:, 2:5], ro[:, 2:5], rtol=1e-11)
+    assert np.allclose(rd[:, 7:], ro[:, 7:], rtol=1e-8, atol=1e-12 * np.abs(ro[:, 7:]).max())
+
+
+def test_propagation_multigrid_2d(ctx):
+    n, dx = 64, 0.3
+    d = T.grid_desc(2, n, dx, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                    absorber=T.ABS_CAP, nCAP=8, multigrid=1)
+    psi0 = crand(n * n) * 0.01
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    pdesc = T.prop_desc(T.MODE_RE, 2, 0.02, gauge=T.GAUGE_VELOCITY,
+                        outputs=T.OUT_NORM | T.OUT_FLUX, borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+    for exec_mode in (0, 1):
+        dev, ref, pd, po, rd, ro = run_both(ctx, d, pdesc, f, psi0, 8, exec_mode)
+        for r in range(4):
+            assert rel(dev[r].psi(), ref[r].psi()) < FP64_TOL, f"exec {exec_mode} region {r}"
+        assert np.allclose(rd[:, 2], ro[:, 2], rtol=1e-11)
+
+
+def test_imaginary_time_ground_state(ctx):
+    n, dx = 256, 0.2
+    d = T.grid_desc(1, n, dx, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1)
+    x = -0.5 * dx * (n - 1) + dx * np.arange(n)
+    psi0 = np.exp(-x * x / 2) + 0j
+    pdesc = T.prop_desc(T.MODE_IM, 1, 0.05, max_steps=200,
+                        outputs=T.OUT_NORMALIZE | T.OUT_NORM | T.OUT_EPOT | T.OUT_EKIN | T.OUT_ETOT
+                        | T.OUT_EDIFF)
+    dev, ref, pd, po, rd, ro = run_both(ctx, d, pdesc, [], psi0, None)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    assert np.allclose(rd[:, 2:7], ro[:, 2:7], rtol=1e-10, atol=1e-13)
+    assert abs(rd[-1, T.REC_ETOT] - (-0.6698)) < 2e-3
+
+
+def test_fp32_mixed_step_close_to_fp64(ctx):
+    n, dx = 128, 0.2
+    base = dict(potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                absorber=T.ABS_CAP, nCAP=16)
+    psi0 = crand(n * n) * 0.01
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.02, gauge=T.GAUGE_VELOCITY, outputs=T.OUT_NORM)
+    from qsf_amd import hip
+    out = []
+    for prec in (T.FP64, T.FP32_MIXED):
+        g = hip.GridSet(ctx, T.grid_desc(2, n, dx, precision=prec, **base))
+        g[0].set_psi(psi0)
+        p = hip.Propagator(g, pdesc, f)
+        rec = p.run(50)
+        out.append((g[0].psi(), rec))
+    assert rel(out[1][0], out[0][0]) < 1e-4
+    assert np.allclose(out[1][1][:, 2], out[0][1][:, 2], rtol=1e-5)
