@@ -1,0 +1,183 @@
+"""Benchmark of the split-operator step on one MI355X.
+
+Default workload (the configuration the headline metric is quoted on): the 2D two-electron
+reduced model, 2048x2048, dx=0.2, Eberly interaction (Z=2, soft-core 0.5 / 0.33), velocity gauge,
+MultiProductSplit<Split3Base<X,P,X>,2> (the reference's "4th order" split, which returns two
+Strang steps of dt/2), dt=0.02, 800 nm sin^2 pulse of 3 cycles with E0=0.12 on both electrons,
+cos^(1/8) absorber over the outer 10% (205 nodes), multigrid regions NO/X/Y/XY with their masks
+and transfers every step, and FLUX<ZOA_FLUX_2D> plus the norm every step. Synthetic initial state
+(seeded complex Gaussian noise on the main region, zero elsewhere).
+
+One "step" = one loop body of SplitPropagator::run (makeStep + computeEach + postCompute) over
+all four region grids. value = grid-point-steps/s of the main grid (value_all_regions counts
+the four region grids). The CPU baseline is the FP64 oracle on a bounded sample of the same
+workload, timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from qsf_amd import qsf_types as T  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_desc(precision, multigrid=1, n=2048):
+    return T.grid_desc(2, n, 0.2, precision=precision, potential=T.POT_EBERLY, Ncharge=2,
+                       Echarge=-1, Nsoft=0.5, Esoft=0.33, absorber=T.ABS_COS18,
+                       nCAP=int(round(0.1 * n)), absorber_tau=0.02, multigrid=multigrid)
+
+
+def c2_prop():
+    return T.prop_desc(T.MODE_RE, 2, 0.02, gauge=T.GAUGE_VELOCITY,
+                       outputs=T.OUT_NORM | T.OUT_FLUX, borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+
+
+C2_FIELDS = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+
+
+def c3_desc(precision, n=512):
+    return T.grid_desc(3, n, 0.3, precision=precision, potential=T.POT_ECKHARDT_SACHA, Ncharge=3,
+                       Echarge=-1, Nsoft=1.0, Esoft=1.0, absorber=T.ABS_CAP, nCAP=32)
+
+
+def c3_prop():
+    return T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_LENGTH,
+                       outputs=T.OUT_NORM | T.OUT_FLUX, borders=[(T.BORDER_BOX, 224)])
+
+
+def c3_fields():
+    E0 = float(np.random.default_rng(3).uniform(0.05, 0.15))
+    return [T.field_desc(T.AXIS_XYZ, E0, 0.057, 2)]
+
+
+def initial_state(m, seed=0):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(size=m) + 1j * rng.normal(size=m)) * 1e-3
+
+
+def run_device(desc, pdesc, fields, steps, warmup, profile=True):
+    from qsf_amd import hip
+    ctx = hip.Context(0)
+    gset = hip.GridSet(ctx, desc)
+    gset[0].set_psi(initial_state(gset[0].m))
+    prop = hip.Propagator(gset, pdesc, fields)
+    prop.start()
+    prop.run_steps(warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.mark()
+    done = prop.run_steps(steps)
+    ctx.mark()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = ctx.elapsed_ms()
+    kernels = {}
+    if profile:
+        # live per-kernel timing over the same number of steps (HIP events on the stream)
+        ctx.profile_begin()
+        prop.run_steps(steps)
+        kernels = ctx.profile_end()
+    rec = prop.records()
+    m_main = gset[0].m
+    m_all = sum(g.m for g in gset.grids)
+    prop.close()
+    gset.close()
+    ctx.close()
+    return dict(done=done, wall=wall, dev_ms=dev_ms, m_main=m_main, m_all=m_all, kernels=kernels,
+                last=rec[-1].tolist() if len(rec) else None)
+
+
+def roofline_of(kernels):
+    if not kernels:
+        return None
+    name, (k, ms, by) = max(kernels.items(), key=lambda kv: kv[1][1])
+    avg_ms = ms / k
+    achieved = by / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(by),
+            "launches": int(k)}
+
+
+def cpu_baseline(desc, pdesc, fields, budget_s=20.0):
+    """The FP64 oracle (one host core) on whole steps of the same workload, bounded in time."""
+    from oracle import oracle as O
+    gs = O.GridSet(desc)
+    gs[0].set_psi(initial_state(gs[0].m))
+    prop = O.Propagator(gs, pdesc, fields)
+    prop.start()
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < 1 or (time.perf_counter() - t0 < budget_s / 2 and steps < 50):
+        prop.step()
+        steps += 1
+    dt = time.perf_counter() - t0
+    m_main = gs[0].m
+    return {"value": round(m_main * steps / dt, 1), "unit": "grid-point-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} whole step(s) of the same workload (all {len(gs)} region grids, "
+                      f"observables every step) on one host core in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every solve lives whole on one MI355X; --gpus {args.gpus} "
+              "is not supported in this build", file=sys.stderr)
+        sys.exit(2)
+
+    if args.workload == "c2":
+        desc, pdesc, fields = c2_desc(T.FP64), c2_prop(), C2_FIELDS
+        workload = ("2D two-electron reduced model 2048x2048 dx=0.2, Eberly, velocity gauge, "
+                    "MultiProductSplit order 2, dt=0.02, 3-cycle 800 nm sin^2 E0=0.12, cos^(1/8) "
+                    "absorber, multigrid NO/X/Y/XY, ZOA_FLUX_2D + norm every step")
+        pts_bytes = 64 * 2   # two P substeps per step survive (last MPS chain), 2D FP64
+    else:
+        desc, pdesc, fields = c3_desc(T.FP64), c3_prop(), c3_fields()
+        workload = ("3D three-electron Eckhardt-Sacha 512^3 dx=0.3, Z=3, length gauge, Strang, "
+                    "dt=0.05, 2-cycle 800 nm, CAP 32 nodes, BOX<224> flux + norm every step")
+        pts_bytes = 128
+
+    r = run_device(desc, pdesc, fields, args.steps, args.warmup)
+    ms_per_step = r["wall"] / r["done"] * 1e3
+    value = r["m_main"] * r["done"] / r["wall"]
+    out = {
+        "metric": "grid-point-steps/s", "value": round(value, 1), "unit": "grid-point-steps/s",
+        "n_gpus": 1, "steps": r["done"], "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128 (f64)",
+        "data": "synthetic", "config": {"workload": workload, "grid_points_main": r["m_main"],
+                                        "grid_points_all_regions": r["m_all"]},
+        "value_all_regions": round(r["m_all"] * r["done"] / r["wall"], 1),
+        "device_ms_per_step": round(r["dev_ms"] / r["done"], 4),
+        "step_roofline": {"bytes_per_grid_point_step": pts_bytes,
+                          "frac": round(value * pts_bytes / (HBM_PEAK_GBS * 1e9), 4)},
+        "roofline": roofline_of(r["kernels"]),
+        "kernels": {k: {"launches": v[0], "avg_us": round(v[1] / v[0] * 1e3, 2)} for k, v in r["kernels"].items()},
+    }
+    if not args.no_variants:
+        if args.workload == "c2":
+            m = run_device(c2_desc(T.FP32_MIXED), pdesc, fields, args.steps, args.warmup, profile=False)
+            out["fp32_mixed"] = {"value": round(m["m_main"] * m["done"] / m["wall"], 1),
+                                 "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, pdesc, fields)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,14 @@ int qsf_ctx_synchronize(qsf_ctx* ctx);
 int qsf_timer_mark(qsf_ctx* ctx);
 int qsf_timer_elapsed_ms(qsf_ctx* ctx, double* ms);
 
+/* Per-kernel timing with HIP events recorded on the context stream around every axis-pass
+ * launch between qsf_prof_begin and qsf_prof_end. Entries are keyed by kernel
+ * ("pass_<prec>_<row|col>_n<N>"); bytes = algorithmic HBM bytes per launch (state read + write). */
+int qsf_prof_begin(qsf_ctx* ctx);
+int qsf_prof_end(qsf_ctx* ctx, int* nentries);
+int qsf_prof_entry(qsf_ctx* ctx, int i, char* name, int name_len, int64_t* launches, double* total_ms,
+                   double* bytes_per_launch);
+
 /* ---- grid sets: one grid, or the 2^D regions of MultiRegionsReduced (mpi_logic.h:246-263) ---- */
 /* Replaces LocalGrid(Section&) construction (grid.h:115-123, multigrid.h:45-46, 124-202). */
 int qsf_set_create(qsf_ctx* ctx, const qsf_grid_desc* desc, qsf_set** out);

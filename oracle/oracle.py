@@ -139,8 +139,11 @@ class Grid:
     def fourier(self, rep):
         lib().orc_fourier(self.h, rep)
 
-    def evolve(self, mode, rep, delta, coupling=None):
-        c = None if coupling is None else _dp(np.ascontiguousarray(coupling, dtype=np.float64))
+    def evolve(self, mode, rep, delta, coupling=None, couples_in=None):
+        # the coupling acts only in the representation it couples in (hamiltonian.h:44-50)
+        if couples_in is not None and couples_in != rep:
+            coupling = None
+        c = None
         if coupling is not None:
             self._keep = np.ascontiguousarray(coupling, dtype=np.float64)
             c = _dp(self._keep)

@@ -59,12 +59,12 @@ struct FFT1D {
     if (m % 2 == 0) return 2;
     for (ind r : {3, 5, 7})
       if (m % r == 0) return r;
-    for (ind r = 11; r * r <= m; r += 2)
+    for (ind r = 11; r <= 13; r += 2)
       if (m % r == 0) return r;
-    return m;
+    return m;  // prime length beyond 13: direct DFT (m == 1 branch)
   }
   // out[0..len) = DFT of in[0], in[s], ..., in[(len-1)s]; scale = n/len for the root table.
-  void rec(const cxd* in, cxd* out, ind len, ind s, int sign, std::vector<cxd>& tmp) const {
+  void rec(const cxd* in, cxd* out, ind len, ind s, int sign) const {
     if (len == 1) { out[0] = in[0]; return; }
     ind r = pick_radix(len);
     ind m = len / r;
@@ -77,8 +77,29 @@ struct FFT1D {
       }
       return;
     }
-    for (ind q = 0; q < r; ++q) rec(in + q * s, out + q * m, m, s * r, sign, tmp);
-    std::vector<cxd> t(r), y(r);
+    for (ind q = 0; q < r; ++q) rec(in + q * s, out + q * m, m, s * r, sign);
+    if (r == 2) {  // X[k] = A[k] + w^k B[k], X[k+m] = A[k] - w^k B[k]
+      for (ind k = 0; k < m; ++k) {
+        cxd a = out[k], b = out[m + k] * w(scale * k, sign);
+        out[k] = a + b;
+        out[m + k] = a - b;
+      }
+      return;
+    }
+    if (r == 4) {
+      const cxd j = sign < 0 ? cxd(0, -1) : cxd(0, 1);  // w_4
+      for (ind k = 0; k < m; ++k) {
+        cxd a = out[k], b = out[m + k] * w(scale * k, sign), c = out[2 * m + k] * w(scale * 2 * k, sign),
+            d = out[3 * m + k] * w(scale * 3 * k, sign);
+        cxd s0 = a + c, s1 = a - c, s2 = b + d, s3 = j * (b - d);
+        out[k] = s0 + s2;
+        out[m + k] = s1 + s3;
+        out[2 * m + k] = s0 - s2;
+        out[3 * m + k] = s1 - s3;
+      }
+      return;
+    }
+    cxd t[16], y[16];
     for (ind k = 0; k < m; ++k) {
       for (ind q = 0; q < r; ++q) t[q] = out[q * m + k] * w(scale * q * k, sign);
       for (ind p = 0; p < r; ++p) {
@@ -90,8 +111,8 @@ struct FFT1D {
     }
   }
   void exec(cxd* data, int sign) const {
-    std::vector<cxd> in(data, data + n), tmp;
-    rec(in.data(), data, n, 1, sign, tmp);
+    std::vector<cxd> in(data, data + n);
+    rec(in.data(), data, n, 1, sign);
   }
 };
 

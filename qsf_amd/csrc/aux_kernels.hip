@@ -9,7 +9,7 @@
 namespace qsf {
 
 struct AxisTableBatch {
-  AxisTableArgs a[12];
+  AxisTableArgs a[24];
 };
 
 template <typename Real>
@@ -54,9 +54,9 @@ __global__ void axis_tables_kernel(AxisTableBatch batch) {
 }
 
 void launch_axis_tables(int precision, const AxisTableArgs* a, int count, hipStream_t s) {
-  for (int off = 0; off < count; off += 12) {
+  for (int off = 0; off < count; off += 24) {
     AxisTableBatch b;
-    int k = count - off < 12 ? count - off : 12;
+    int k = count - off < 24 ? count - off : 24;
     int64_t nmax = 0;
     for (int i = 0; i < k; ++i) {
       b.a[i] = a[off + i];
@@ -180,12 +180,22 @@ __global__ void flux_kernel(FluxArgs a) {
 
 __global__ void flux_finish_kernel(const double* partials, int64_t count, int nb, double scale, double* out,
                                    double* accum, double w) {
-  if (threadIdx.x != 0) return;
+  // one family at a time: the block loads the per-block sums and folds them in a fixed tree
+  __shared__ double red[256];
   for (int b = 0; b < nb; ++b) {
-    double s = 0.0;
-    for (int64_t k = 0; k < count; ++k) s += partials[k * nb + b];
-    out[b] = s * scale;
-    if (accum) accum[b] += out[b] * w;
+    double acc = 0.0;
+    for (int64_t k = threadIdx.x; k < count; k += blockDim.x) acc += partials[k * nb + b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      out[b] = red[0] * scale;
+      if (accum) accum[b] += out[b] * w;
+    }
+    __syncthreads();
   }
 }
 

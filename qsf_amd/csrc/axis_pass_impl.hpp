@@ -30,14 +30,16 @@ template <typename Real, int N, bool ROW> struct Geo {
   static constexpr int lds_cap(int w) { return w * N * CB; }
   static constexpr int pick() {
     if (ROW) {
-      int w = 256 / T;
+      // at least 128 threads per block, at most 32 KiB of LDS: several blocks per CU overlap
+      // their load, transform and store phases
+      int w = 128 / T;
       if (w < 1) w = 1;
-      while (w > 1 && lds_cap(w) > 65536) w /= 2;
+      while (w > 1 && lds_cap(w) > 32768) w /= 2;
       return w;
     }
+    // columns: W adjacent columns per block; 64 KiB LDS keeps two blocks per CU
     int w = 16;
-    const int budget = (N * CB >= 32768) ? 131072 : 65536;
-    while (w > 1 && (lds_cap(w) > budget || w * T > 1024)) w /= 2;
+    while (w > 1 && (lds_cap(w) > 65536 || w * T > 1024)) w /= 2;
     return w;
   }
   static constexpr int W = pick();
@@ -58,34 +60,39 @@ __device__ __forceinline__ T ld(const void* tab, int64_t i) {
   return reinterpret_cast<const T*>(tab)[i];
 }
 
-// Per-thread evaluator of an X factor for the elements of one transform along `axis`.
+// Per-thread evaluator of an X factor for the elements of one transform along `axis`
+// (scalar members only: runtime-indexed arrays would live in scratch memory).
 template <typename T, int N> struct XEval {
   T cst;
   const T* ax;
-  const T* pr[2];
-  int other[2];
-  int npr;
-  int kind;
-  __device__ void init(const XFactor& f, int axis, const int* idx) {
+  const T* pr0;
+  const T* pr1;
+  int o0, o1, npr, kind;
+  __device__ void init(const XFactor& f, int axis, int i0, int i1, int i2) {
     using R = decltype(cst.x);
+    auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
     cst = T{R(1), R(0)};
-    ax = reinterpret_cast<const T*>(f.axis[axis]);
+    ax = reinterpret_cast<const T*>(axis == 0 ? f.axis[0] : (axis == 1 ? f.axis[1] : f.axis[2]));
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-      if (d != axis && f.axis[d]) cst = cmul(cst, ld<T>(f.axis[d], idx[d]));
+      if (d != axis && f.axis[d]) cst = cmul(cst, ld<T>(f.axis[d], id(d)));
     npr = 0;
+    o0 = o1 = 0;
+    pr0 = pr1 = nullptr;
     kind = f.pair_kind;
     if (kind != PAIR_NONE) {
-      const int P[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (!(f.pair_mask & (1 << k))) continue;
-        int p = P[k][0], q = P[k][1];
+        const int p = k, q = (k + 1) % 3;   // pairs (0,1), (1,2), (2,0)
         if (p != axis && q != axis) {
-          int64_t i = kind == PAIR_DIFF ? (int64_t)abs(idx[p] - idx[q]) : (int64_t)idx[p] * f.pair_n + idx[q];
+          int64_t i = kind == PAIR_DIFF ? (int64_t)abs(id(p) - id(q)) : (int64_t)id(p) * f.pair_n + id(q);
           cst = cmul(cst, ld<T>(f.pair, i));
         } else {
-          int o = (p == axis) ? idx[q] : idx[p];
-          pr[npr] = reinterpret_cast<const T*>(f.pair) + (kind == PAIR_FULL ? (int64_t)o * f.pair_n : 0);
-          other[npr] = o;
+          const int o = (p == axis) ? id(q) : id(p);
+          const T* base = reinterpret_cast<const T*>(f.pair) + (kind == PAIR_FULL ? (int64_t)o * f.pair_n : 0);
+          if (npr == 0) { pr0 = base; o0 = o; }
+          else { pr1 = base; o1 = o; }
           ++npr;
         }
       }
@@ -94,10 +101,8 @@ template <typename T, int N> struct XEval {
   __device__ __forceinline__ T at(int n) const {
     T r = cst;
     if (ax) r = cmul(r, ax[n]);
-    for (int k = 0; k < npr; ++k) {
-      T p = (kind == PAIR_DIFF) ? pr[k][abs(n - other[k])] : pr[k][n];
-      r = cmul(r, p);
-    }
+    if (npr > 0) r = cmul(r, kind == PAIR_DIFF ? pr0[abs(n - o0)] : pr0[n]);
+    if (npr > 1) r = cmul(r, kind == PAIR_DIFF ? pr1[abs(n - o1)] : pr1[n]);
     return r;
   }
 };
@@ -171,17 +176,19 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
   const int64_t base = ROW ? o * N : o * (int64_t)N * p.inner + c;
   const int64_t step = ROW ? 1 : p.inner;
 
-  int idx[3] = {0, 0, 0};
+  // coordinates of this thread's transform (the transform axis index is per element)
+  int i0 = 0, i1 = 0, i2 = 0;
   if (p.axis == 0) {
-    idx[1] = (int)(c / p.n[2]);
-    idx[2] = (int)(c % p.n[2]);
+    i1 = (int)(c / p.n[2]);
+    i2 = (int)(c % p.n[2]);
   } else if (p.axis == 1) {
-    idx[0] = (int)o;
-    idx[2] = (int)c;
+    i0 = (int)o;
+    i2 = (int)c;
   } else {
-    idx[0] = (int)(o / p.n[1]);
-    idx[1] = (int)(o % p.n[1]);
+    i0 = (int)(o / p.n[1]);
+    i1 = (int)(o % p.n[1]);
   }
+  auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
 
   const T* roots = reinterpret_cast<const T*>(p.roots);
   const T* src = reinterpret_cast<const T*>(p.src);
@@ -199,10 +206,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
     constexpr int DIR = decltype(dir_tag)::value;
     if constexpr (ROW) {
       RowAddr<T> a{w * N};
-      fft<N, DIR>(v, t, lds, a, roots);
+      fft<N, DIR>(v, t, lds, a, roots, p.tw_mode);
     } else {
       ColAddr<W> a{w};
-      fft<N, DIR>(v, t, lds, a, roots);
+      fft<N, DIR>(v, t, lds, a, roots, p.tw_mode);
     }
   };
   using FWD = std::integral_constant<int, -1>;
@@ -212,8 +219,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
 
   if (p.flags & F_XA) {
     XEval<T, N> xe;
-    idx[p.axis] = 0;
-    xe.init(p.xa, p.axis, idx);
+    xe.init(p.xa, p.axis, i0, i1, i2);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cmul(v[q], xe.at(t + TT * q));
   }
@@ -222,19 +228,22 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
     const ObsParams& ob = p.obs;
     double nrm = 0.0, ep = 0.0, ek = 0.0;
     if (valid) {
-      double co[3];
       const int D = p.n[2] > 1 ? 3 : (p.n[1] > 1 ? 2 : 1);
-      for (int d = 0; d < 3; ++d) co[d] = ob.xmin[d] + idx[d] * ob.dx[d];
       double kc = 0.0;
+#pragma unroll
       for (int d = 0; d < 3; ++d)
-        if (d != p.axis && ob.kin_w[d]) kc += ob.kin_w[d][idx[d]];
-      const double* ka = ob.kin_w[p.axis];
+        if (d != p.axis && ob.kin_w[d]) kc += ob.kin_w[d][id(d)];
+      const double* ka = p.axis == 0 ? ob.kin_w[0] : (p.axis == 1 ? ob.kin_w[1] : ob.kin_w[2]);
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         double a2 = (double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y;
         nrm += a2;
         if (ob.want_epot) {
-          co[p.axis] = ob.xmin[p.axis] + (t + TT * q) * ob.dx[p.axis];
+          const int ia = t + TT * q;
+          double co[3];
+          co[0] = ob.xmin[0] + (p.axis == 0 ? ia : i0) * ob.dx[0];
+          co[1] = ob.xmin[1] + (p.axis == 1 ? ia : i1) * ob.dx[1];
+          co[2] = ob.xmin[2] + (p.axis == 2 ? ia : i2) * ob.dx[2];
           ep += a2 * pot_value(ob, co, D);
         }
         if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[t + TT * q] : 0.0));
@@ -283,24 +292,27 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
   if (p.flags & F_MASK) {
     // maskRegion (grid.h:582-597): one exact FP64 factor per point, from per-axis CAP nodes
     int64_t mo = 0;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (d == p.axis || p.mask_stride[d] == 0) continue;
-      const int64_t h = p.n[d] / 2, i = idx[d];
+      const int64_t h = p.n[d] / 2, i = id(d);
       const int64_t nd = i < h ? -i : i - p.n[d] + 1;
       int64_t node = p.nCAP + nd;
       node = node < 0 ? 0 : node;
       mo += node * p.mask_stride[d];
     }
-    const int64_t ha = p.n[p.axis] / 2;
+    const int64_t na = p.axis == 0 ? p.n[0] : (p.axis == 1 ? p.n[1] : p.n[2]);
+    const int64_t ha = na / 2;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int64_t off = mo;
-      if (p.mask_stride[p.axis]) {
+      const int64_t msa = p.axis == 0 ? p.mask_stride[0] : (p.axis == 1 ? p.mask_stride[1] : p.mask_stride[2]);
+      if (msa) {
         const int64_t i = t + TT * q;
-        const int64_t nd = i < ha ? -i : i - p.n[p.axis] + 1;
+        const int64_t nd = i < ha ? -i : i - na + 1;
         int64_t node = p.nCAP + nd;
         node = node < 0 ? 0 : node;
-        off += node * p.mask_stride[p.axis];
+        off += node * msa;
       }
       const double mk = p.mask_tab[off];
       v[q] = T{(Real)((double)v[q].x * mk), (Real)((double)v[q].y * mk)};
@@ -309,8 +321,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
 
   if (p.flags & F_XB) {
     XEval<T, N> xe;
-    idx[p.axis] = 0;
-    xe.init(p.xb, p.axis, idx);
+    xe.init(p.xb, p.axis, i0, i1, i2);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cmul(v[q], xe.at(t + TT * q));
   }
@@ -319,9 +330,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
 
   if (p.flags & F_P) {
     T cst{Real(1), Real(0)};
+#pragma unroll
     for (int d = 0; d < 3; ++d)
-      if (d != p.axis && p.pf[d]) cst = cmul(cst, ld<T>(p.pf[d], idx[d]));
-    const T* pa = reinterpret_cast<const T*>(p.pf[p.axis]);
+      if (d != p.axis && p.pf[d]) cst = cmul(cst, ld<T>(p.pf[d], id(d)));
+    const T* pa = reinterpret_cast<const T*>(p.axis == 0 ? p.pf[0] : (p.axis == 1 ? p.pf[1] : p.pf[2]));
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       T f = pa ? cmul(cst, pa[t + TT * q]) : cst;

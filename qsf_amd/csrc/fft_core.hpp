@@ -176,7 +176,8 @@ template <int N> struct PlanOf {
 // v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
 // thread's transform through addr(p). `roots[j] = exp(-2πi j/N)` (N entries).
 template <int N, int DIR, int S, typename T, typename Addr>
-__device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots) {
+__device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
+                                          int tw_mode) {
   using PL = PlanOf<N>;
   constexpr int E = PL::E, TT = PL::T;
   constexpr int R = PL::template R<S>;
@@ -192,11 +193,42 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
     if constexpr (NS > 1) {
+      if (tw_mode == 0) {
+        // every twiddle from the FP64-built root table
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        T w = roots[r * k * TWS];
-        if constexpr (DIR > 0) w = cconj(w);
-        x[r] = cmul(x[r], w);
+        for (int r = 1; r < R; ++r) {
+          T w = roots[r * k * TWS];
+          if constexpr (DIR > 0) w = cconj(w);
+          x[r] = cmul(x[r], w);
+        }
+      } else {
+        // two table loads (w^k, w^4k); the other powers by at most three products
+        T w1 = roots[k * TWS];
+        if constexpr (DIR > 0) w1 = cconj(w1);
+        T w4 = w1;
+        if constexpr (R > 4) {
+          w4 = roots[4 * k * TWS];
+          if constexpr (DIR > 0) w4 = cconj(w4);
+        }
+        T w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        T pw[4] = {T{}, w1, w2, w3};
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          T w;
+          if (r < 4) w = pw[r];
+          else if (r == 4) w = w4;
+          else if (r < 8) w = cmul(w4, pw[r - 4]);
+          else {
+            T w8 = cmul(w4, w4);
+            if (r == 8) w = w8;
+            else if (r < 12) w = cmul(w8, pw[r - 8]);
+            else {
+              T w12 = cmul(w8, w4);
+              w = (r == 12) ? w12 : cmul(w12, pw[r - 12]);
+            }
+          }
+          x[r] = cmul(x[r], w);
+        }
       }
     }
     dft<R, DIR>(x);
@@ -218,16 +250,19 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 }
 
 template <int N, int DIR, int S, typename T, typename Addr>
-__device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots) {
+__device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
+                                           int tw_mode) {
   if constexpr (S < PlanOf<N>::NST) {
-    fft_stage<N, DIR, S>(v, t, lds, addr, roots);
-    fft_stages<N, DIR, S + 1>(v, t, lds, addr, roots);
+    fft_stage<N, DIR, S>(v, t, lds, addr, roots, tw_mode);
+    fft_stages<N, DIR, S + 1>(v, t, lds, addr, roots, tw_mode);
   }
 }
 
+// tw_mode 0: all twiddles from the table; 1: two loads and products per butterfly.
 template <int N, int DIR, typename T, typename Addr>
-__device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots) {
-  fft_stages<N, DIR, 0>(v, t, lds, addr, roots);
+__device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
+                                    int tw_mode) {
+  fft_stages<N, DIR, 0>(v, t, lds, addr, roots, tw_mode);
 }
 
 }  // namespace qsf

@@ -58,6 +58,7 @@ struct PassParams {
   int64_t outer;         // prod_{d<axis} n_d
   const void* roots;     // N roots exp(-2πi j/N)
   uint32_t flags;
+  int32_t tw_mode;       // twiddles: 0 = table load each, 1 = two loads + products
   double scale;
   XFactor xa, xb;
   const void* pf[3];     // P factor tables (n_a entries; nullptr = 1)

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -65,9 +66,38 @@ struct qsf_ctx {
   std::string err;
   hipEvent_t ev[2] = {nullptr, nullptr};
   int marks = 0;
+  // live per-kernel timing (qsf_prof_*)
+  bool prof = false;
+  struct Launch { std::string key; hipEvent_t a, b; double bytes; };
+  std::vector<Launch> launches;
+  std::vector<hipEvent_t> spare;
+  struct Entry { std::string key; int64_t n; double ms, bytes; };
+  std::vector<Entry> entries;
+  int tw_mode[2] = {1, 0};  // FP64: two loads + products; FP32: table loads
+  hipEvent_t event() {
+    if (!spare.empty()) { hipEvent_t e = spare.back(); spare.pop_back(); return e; }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
 };
 
 namespace qsf {
+
+// Fold recorded launch events into per-kernel totals (synchronises the stream).
+inline void drain_prof(qsf_ctx* c) {
+  QSF_HIP(hipStreamSynchronize(c->stream));
+  for (auto& l : c->launches) {
+    float ms = 0;
+    QSF_HIP(hipEventElapsedTime(&ms, l.a, l.b));
+    auto it = std::find_if(c->entries.begin(), c->entries.end(), [&](const qsf_ctx::Entry& e) { return e.key == l.key; });
+    if (it == c->entries.end()) c->entries.push_back({l.key, 1, ms, l.bytes});
+    else { it->n += 1; it->ms += ms; }
+    c->spare.push_back(l.a);
+    c->spare.push_back(l.b);
+  }
+  c->launches.clear();
+}
 
 // ----------------------------------------------------------------------------------------
 // Grid
@@ -93,6 +123,9 @@ struct Grid {
   std::map<int64_t, std::unique_ptr<DBuf>> roots;
   std::map<std::pair<int64_t, int>, std::unique_ptr<DBuf>> pairs;
   std::unique_ptr<Gather> gather;
+  // per-substep phase tables requested but not yet built: they point into `tabs`, so they
+  // live and die with this grid; one launch builds them before the next pass
+  std::vector<AxisTableArgs> pending_tables;
   int rep = QSF_REP_X;
   int slot = 0;
   static constexpr int kSlots = 48;
@@ -301,7 +334,7 @@ struct Grid {
       t.tau = d.absorber_tau;
       t.norm = 1.0;
     }
-    launch_axis_tables(prec, args, D, st());
+    for (int a = 0; a < D; ++a) pending_tables.push_back(args[a]);
     XFactor f{};
     for (int a = 0; a < D; ++a) f.axis[a] = args[a].out;
     f.pair_mask = pair_mask();
@@ -335,7 +368,7 @@ struct Grid {
       const bool fa = std::find(fftAxes.begin(), fftAxes.end(), a) != fftAxes.end();
       t.norm = (normalise && fa) ? 1.0 / double(n[a]) : 1.0;
     }
-    launch_axis_tables(prec, args, D, st());
+    for (int a = 0; a < D; ++a) pending_tables.push_back(args[a]);
     for (int a = 0; a < 3; ++a) pf[a] = a < D ? args[a].out : nullptr;
   }
 
@@ -367,15 +400,37 @@ struct Grid {
     o.want_epot = epot;
   }
 
+  void build_tables() {
+    if (!pending_tables.empty()) launch_axis_tables(prec, pending_tables.data(), (int)pending_tables.size(), st());
+    pending_tables.clear();
+  }
+
   int run(PassParams& p) {
+    build_tables();
+    p.tw_mode = ctx->tw_mode[prec];
     if (p.flags & F_OBS) {
       int64_t tiles = axis_pass_tiles(prec, p);
       partials.ensure(tiles * 3 * sizeof(double));
       if (!p.obs.partials) p.obs.partials = partials.as<double>();
     }
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (ctx->prof) {
+      ea = ctx->event();
+      eb = ctx->event();
+      QSF_HIP(hipEventRecord(ea, st()));
+    }
     int k = launch_axis_pass(prec, p, st());
     QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
     QSF_HIP(hipGetLastError());
+    if (ctx->prof) {
+      QSF_HIP(hipEventRecord(eb, st()));
+      const bool row = p.inner == 1;
+      std::string key = std::string("pass_") + (prec == QSF_FP64 ? "f64" : "f32") + (row ? "_row_n" : "_col_n") +
+                        std::to_string(n[p.axis]);
+      const double bytes = (double)m * cb * ((p.flags & F_NOSTORE) ? 1.0 : 2.0);
+      ctx->launches.push_back({key, ea, eb, bytes});
+      if (ctx->launches.size() >= 4096) drain_prof(ctx);
+    }
     return k;
   }
 
@@ -1166,6 +1221,7 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
   if (!out) return QSF_ERR_ARG;
   auto* c = new qsf_ctx();
   c->device = device;
+  if (const char* e = std::getenv("QSF_TW_MODE")) c->tw_mode[0] = c->tw_mode[1] = std::atoi(e);
   int rc = guard(nullptr, [&] {
     int count = 0;
     QSF_HIP(hipGetDeviceCount(&count));
@@ -1184,9 +1240,42 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
   return QSF_OK;
 }
 
+int qsf_prof_begin(qsf_ctx* c) {
+  if (!c) return QSF_ERR_ARG;
+  return guard(c, [&] {
+    drain_prof(c);
+    c->entries.clear();
+    c->prof = true;
+  });
+}
+
+int qsf_prof_end(qsf_ctx* c, int* n) {
+  if (!c || !n) return QSF_ERR_ARG;
+  return guard(c, [&] {
+    drain_prof(c);
+    c->prof = false;
+    *n = (int)c->entries.size();
+  });
+}
+
+int qsf_prof_entry(qsf_ctx* c, int i, char* name, int len, int64_t* k, double* ms, double* bytes) {
+  if (!c || i < 0 || i >= (int)c->entries.size()) return QSF_ERR_ARG;
+  const auto& e = c->entries[i];
+  if (name && len > 0) {
+    std::strncpy(name, e.key.c_str(), len - 1);
+    name[len - 1] = 0;
+  }
+  if (k) *k = e.n;
+  if (ms) *ms = e.ms;
+  if (bytes) *bytes = e.bytes;
+  return QSF_OK;
+}
+
 int qsf_ctx_destroy(qsf_ctx* c) {
   if (!c) return QSF_ERR_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& l : c->launches) { (void)hipEventDestroy(l.a); (void)hipEventDestroy(l.b); }
+  for (auto& e : c->spare) (void)hipEventDestroy(e);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

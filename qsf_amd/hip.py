@@ -3,8 +3,10 @@
 The library is the product: every numeric operation below runs in its HIP kernels. Loading
 fails loudly when the library is missing; there is no CPU fallback.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -55,6 +57,9 @@ SIGNATURES = {
     "qsf_prop_records": (_I, [_P, _dp, _I]),
     "qsf_prop_flux_integral": (_I, [_P, _dp]),
     "qsf_prop_set_exec": (_I, [_P, _I]),
+    "qsf_prof_begin": (_I, [_P]),
+    "qsf_prof_end": (_I, [_P, C.POINTER(_I)]),
+    "qsf_prof_entry": (_I, [_P, _I, C.c_char_p, _I, C.POINTER(_I64), _dp, _dp]),
 }
 
 
@@ -76,6 +81,15 @@ class QSFError(RuntimeError):
     pass
 
 
+_EXITING = [False]
+
+
+@atexit.register
+def _mark_exit():
+    # at interpreter exit the HIP runtime may already be gone: leave handles to the OS
+    _EXITING[0] = True
+
+
 def _dptr(a):
     return a.ctypes.data_as(_dp)
 
@@ -90,6 +104,10 @@ class Context:
         if rc != 0:
             raise QSFError(f"qsf_ctx_create: {L.qsf_last_error(None).decode()}")
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        self._children.add(obj)
 
     def check(self, rc, what):
         if rc != 0:
@@ -102,15 +120,36 @@ class Context:
     def mark(self):
         self.check(load().qsf_timer_mark(self.h), "timer_mark")
 
+    def profile_begin(self):
+        """Record HIP events around every axis-pass launch from now on."""
+        self.check(load().qsf_prof_begin(self.h), "prof_begin")
+
+    def profile_end(self):
+        """Stop recording; return {kernel: (launches, total_ms, algorithmic_bytes_per_launch)}."""
+        n = C.c_int()
+        self.check(load().qsf_prof_end(self.h, C.byref(n)), "prof_end")
+        out = {}
+        buf = C.create_string_buffer(256)
+        for i in range(n.value):
+            k, ms, by = C.c_int64(), C.c_double(), C.c_double()
+            self.check(load().qsf_prof_entry(self.h, i, buf, 256, C.byref(k), C.byref(ms), C.byref(by)),
+                       "prof_entry")
+            out[buf.value.decode()] = (k.value, ms.value, by.value)
+        return out
+
     def elapsed_ms(self):
         v = C.c_double()
         self.check(load().qsf_timer_elapsed_ms(self.h, C.byref(v)), "timer_elapsed")
         return v.value
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not _EXITING[0]:
+            for ch in [c for c in self._children if isinstance(c, Propagator)]:
+                ch.close()
+            for ch in list(self._children):
+                ch.close()
             load().qsf_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -188,6 +227,8 @@ class GridSet:
         h = _P()
         ctx.check(load().qsf_set_create(ctx.h, C.byref(desc), C.byref(h)), "set_create")
         self.h = h
+        ctx.adopt(self)
+        self._props = weakref.WeakSet()
         n = C.c_int()
         ctx.check(load().qsf_set_count(h, C.byref(n)), "set_count")
         self.grids = []
@@ -206,9 +247,12 @@ class GridSet:
         self.ctx.check(load().qsf_set_post_compute(self.h), "post_compute")
 
     def close(self):
-        if getattr(self, "h", None):
+        # propagators hold the set: they go first
+        for p in list(getattr(self, "_props", ())):
+            p.close()
+        if getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None):
             load().qsf_set_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -228,13 +272,15 @@ class Propagator:
         self.ctx.check(load().qsf_prop_create(gset.h, C.byref(pdesc), self._fields, len(fields),
                                               C.byref(h)), "prop_create")
         self.h = h
+        self.ctx.adopt(self)
+        gset._props.add(self)
         if exec_mode:
             self.ctx.check(load().qsf_prop_set_exec(h, exec_mode), "prop_set_exec")
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None):
             load().qsf_prop_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

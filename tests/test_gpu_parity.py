@@ -305,3 +305,21 @@ def test_fp32_mixed_step_close_to_fp64(ctx):
         out.append((g[0].psi(), rec))
     assert rel(out[1][0], out[0][0]) < 1e-4
     assert np.allclose(out[1][1][:, 2], out[0][1][:, 2], rtol=1e-5)
+
+
+def test_grid_teardown_with_unbuilt_tables(ctx):
+    # a propagator stopped mid-run leaves phase tables queued for the next pass; they belong to
+    # their grid and must be dropped with it, not built into freed memory by another grid
+    from qsf_amd import hip
+    base = dict(potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                absorber=T.ABS_CAP, nCAP=16)
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    for prec in (T.FP64, T.FP32_MIXED, T.FP64):
+        g = hip.GridSet(ctx, T.grid_desc(2, 128, 0.2, precision=prec, **base))
+        g[0].set_psi(crand(128 * 128) * 0.01)
+        p = hip.Propagator(g, T.prop_desc(T.MODE_RE, 1, 0.02, outputs=0), f)
+        p.start()
+        p.run_steps(3)
+        p.close()
+        g.close()
+    ctx.synchronize()
