@@ -68,6 +68,8 @@ def run_device(desc, pdesc, fields, steps, warmup, profile=True):
     ctx = hip.Context(0)
     gset = hip.GridSet(ctx, desc)
     gset[0].set_psi(initial_state(gset[0].m))
+    for g in gset.grids:   # pick the fastest compiled kernel variant of every pass (untimed)
+        g.autotune(5)
     prop = hip.Propagator(gset, pdesc, fields)
     prop.start()
     prop.run_steps(warmup)

@@ -50,6 +50,16 @@ int qsf_prof_end(qsf_ctx* ctx, int* nentries);
 int qsf_prof_entry(qsf_ctx* ctx, int i, char* name, int name_len, int64_t* launches, double* total_ms,
                    double* bytes_per_launch);
 
+/* Autotuner: times every compiled variant (tile width x elements per thread) of the axis
+ * passes this grid's step uses (reps launches each, HIP events) and keeps the fastest for
+ * (precision, row/column mode, axis length, grid size) in the context. Entries can be read back
+ * and set again (e.g. from a cache file kept by the caller). */
+int qsf_autotune(qsf_grid* g, int reps);
+int qsf_tune_count(qsf_ctx* ctx, int* count);
+int qsf_tune_entry(qsf_ctx* ctx, int i, int* precision, int* row, int64_t* n, int64_t* grid_size,
+                   int* variant, double* best_ms);
+int qsf_tune_set(qsf_ctx* ctx, int precision, int row, int64_t n, int64_t grid_size, int variant);
+
 /* ---- grid sets: one grid, or the 2^D regions of MultiRegionsReduced (mpi_logic.h:246-263) ---- */
 /* Replaces LocalGrid(Section&) construction (grid.h:115-123, multigrid.h:45-46, 124-202). */
 int qsf_set_create(qsf_ctx* ctx, const qsf_grid_desc* desc, qsf_set** out);

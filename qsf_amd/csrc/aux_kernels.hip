@@ -132,8 +132,34 @@ __global__ void reduce_partials_kernel(const double* partials, int64_t count, in
   }
 }
 
-void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, count, width, out);
+// First level: block b sums the contiguous chunk [b*chunk, (b+1)*chunk) of the partials.
+__global__ void reduce_chunks_kernel(const double* partials, int64_t count, int width, int64_t chunk, double* out) {
+  __shared__ double red[256];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < count ? lo + chunk : count;
+  for (int c = 0; c < width; ++c) {
+    double acc = 0.0;
+    for (int64_t k = lo + threadIdx.x; k < hi; k += 256) acc += partials[k * width + c];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x * width + c] = red[0];
+    __syncthreads();
+  }
+}
+
+void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, double* scratch,
+                            hipStream_t s) {
+  if (count <= 4096 || !scratch) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, count, width, out);
+    return;
+  }
+  // two fixed-order levels: 256 chunk sums into `scratch` (256*width doubles), then one block
+  const int64_t chunk = (count + 255) / 256;
+  hipLaunchKernelGGL(reduce_chunks_kernel, dim3(256), dim3(256), 0, s, partials, count, width, chunk, scratch);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, scratch, (int64_t)256, width, out);
 }
 
 template <typename Real>
@@ -200,10 +226,11 @@ __global__ void flux_finish_kernel(const double* partials, int64_t count, int nb
 }
 
 void launch_flux(const FluxArgs& a, hipStream_t s) {
-  const int blocks = 64;
+  int64_t want = (a.npts + 255) / 256;
+  const int blocks = (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
   if (a.precision == QSF_FP64) hipLaunchKernelGGL(flux_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(flux_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(flux_finish_kernel, dim3(1), dim3(64), 0, s, a.partials, (int64_t)blocks, a.nb, a.scale,
+  hipLaunchKernelGGL(flux_finish_kernel, dim3(1), dim3(256), 0, s, a.partials, (int64_t)blocks, a.nb, a.scale,
                      a.out, a.accum, a.accum_weight);
 }
 

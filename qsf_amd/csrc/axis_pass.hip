@@ -6,14 +6,14 @@
 
 namespace qsf {
 
-int dispatch_f64_row(int64_t n, const PassParams& p, hipStream_t s);
-int dispatch_f64_col(int64_t n, const PassParams& p, hipStream_t s);
-int dispatch_f32_row(int64_t n, const PassParams& p, hipStream_t s);
-int dispatch_f32_col(int64_t n, const PassParams& p, hipStream_t s);
-int64_t tiles_f64_row(int64_t n, const PassParams& p);
-int64_t tiles_f64_col(int64_t n, const PassParams& p);
-int64_t tiles_f32_row(int64_t n, const PassParams& p);
-int64_t tiles_f32_col(int64_t n, const PassParams& p);
+int dispatch_f64_row(int64_t n, int v, const PassParams& p, hipStream_t s);
+int dispatch_f64_col(int64_t n, int v, const PassParams& p, hipStream_t s);
+int dispatch_f32_row(int64_t n, int v, const PassParams& p, hipStream_t s);
+int dispatch_f32_col(int64_t n, int v, const PassParams& p, hipStream_t s);
+int64_t tiles_f64_row(int64_t n, int v, const PassParams& p);
+int64_t tiles_f64_col(int64_t n, int v, const PassParams& p);
+int64_t tiles_f32_row(int64_t n, int v, const PassParams& p);
+int64_t tiles_f32_col(int64_t n, int v, const PassParams& p);
 
 bool axis_size_supported(int64_t n) {
   switch (n) {
@@ -25,18 +25,21 @@ bool axis_size_supported(int64_t n) {
   }
 }
 
-int launch_axis_pass(int precision, const PassParams& p, hipStream_t s) {
+int axis_variants(int64_t n) { return (n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 ||
+                                        n == 768 || n == 1536) ? 6 : 1; }
+
+int launch_axis_pass(int precision, const PassParams& p, hipStream_t s, int v) {
   const int64_t n = p.n[p.axis];
   const bool row = (p.inner == 1);
-  if (precision == 0) return row ? dispatch_f64_row(n, p, s) : dispatch_f64_col(n, p, s);
-  return row ? dispatch_f32_row(n, p, s) : dispatch_f32_col(n, p, s);
+  if (precision == 0) return row ? dispatch_f64_row(n, v, p, s) : dispatch_f64_col(n, v, p, s);
+  return row ? dispatch_f32_row(n, v, p, s) : dispatch_f32_col(n, v, p, s);
 }
 
-int64_t axis_pass_tiles(int precision, const PassParams& p) {
+int64_t axis_pass_tiles(int precision, const PassParams& p, int v) {
   const int64_t n = p.n[p.axis];
   const bool row = (p.inner == 1);
-  if (precision == 0) return row ? tiles_f64_row(n, p) : tiles_f64_col(n, p);
-  return row ? tiles_f32_row(n, p) : tiles_f32_col(n, p);
+  if (precision == 0) return row ? tiles_f64_row(n, v, p) : tiles_f64_col(n, v, p);
+  return row ? tiles_f32_row(n, v, p) : tiles_f32_col(n, v, p);
 }
 
 }  // namespace qsf

@@ -2,6 +2,6 @@
 #include "axis_pass_impl.hpp"
 
 namespace qsf {
-int dispatch_f32_col(int64_t n, const PassParams& p, hipStream_t s) { return dispatch_n<float, false>(n, p, s); }
-int64_t tiles_f32_col(int64_t n, const PassParams& p) { return tiles_n<float, false>(n, p); }
+int dispatch_f32_col(int64_t n, int v, const PassParams& p, hipStream_t s) { return dispatch_n<float, false>(n, v, p, s); }
+int64_t tiles_f32_col(int64_t n, int v, const PassParams& p) { return tiles_n<float, false>(n, v, p); }
 }  // namespace qsf

@@ -2,6 +2,6 @@
 #include "axis_pass_impl.hpp"
 
 namespace qsf {
-int dispatch_f64_col(int64_t n, const PassParams& p, hipStream_t s) { return dispatch_n<double, false>(n, p, s); }
-int64_t tiles_f64_col(int64_t n, const PassParams& p) { return tiles_n<double, false>(n, p); }
+int dispatch_f64_col(int64_t n, int v, const PassParams& p, hipStream_t s) { return dispatch_n<double, false>(n, v, p, s); }
+int64_t tiles_f64_col(int64_t n, int v, const PassParams& p) { return tiles_n<double, false>(n, v, p); }
 }  // namespace qsf

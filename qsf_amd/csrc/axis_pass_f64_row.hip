@@ -2,6 +2,6 @@
 #include "axis_pass_impl.hpp"
 
 namespace qsf {
-int dispatch_f64_row(int64_t n, const PassParams& p, hipStream_t s) { return dispatch_n<double, true>(n, p, s); }
-int64_t tiles_f64_row(int64_t n, const PassParams& p) { return tiles_n<double, true>(n, p); }
+int dispatch_f64_row(int64_t n, int v, const PassParams& p, hipStream_t s) { return dispatch_n<double, true>(n, v, p, s); }
+int64_t tiles_f64_row(int64_t n, int v, const PassParams& p) { return tiles_n<double, true>(n, v, p); }
 }  // namespace qsf

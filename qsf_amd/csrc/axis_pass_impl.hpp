@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "fft_core.hpp"
 #include "pass_params.hpp"
@@ -22,13 +23,19 @@ template <typename Real> struct Tr {
   using T = typename cvec<Real>::T;
 };
 
-// Block geometry per (precision, N, mode).
-template <typename Real, int N, bool ROW> struct Geo {
-  using PL = PlanOf<N>;
+// Kernel variant V (0..5): ALT = V / 3 selects the elements-per-thread plan, V % 3 scales the
+// tile width W (x1, x1/2, x2) within the LDS and block-size limits.
+constexpr int kVariants = 6;
+constexpr int var_alt(int v) { return v / 3; }
+constexpr int var_wshift(int v) { return v % 3 == 0 ? 0 : (v % 3 == 1 ? -1 : 1); }
+
+// Block geometry per (precision, N, mode, variant).
+template <typename Real, int N, bool ROW, int V = 0> struct Geo {
+  using PL = PlanOf<N, var_alt(V)>;
   static constexpr int T = PL::T;
   static constexpr int CB = (int)sizeof(typename cvec<Real>::T);
   static constexpr int lds_cap(int w) { return w * N * CB; }
-  static constexpr int pick() {
+  static constexpr int base() {
     if (ROW) {
       // at least 128 threads per block, at most 32 KiB of LDS: several blocks per CU overlap
       // their load, transform and store phases
@@ -40,6 +47,12 @@ template <typename Real, int N, bool ROW> struct Geo {
     // columns: W adjacent columns per block; 64 KiB LDS keeps two blocks per CU
     int w = 16;
     while (w > 1 && (lds_cap(w) > 65536 || w * T > 1024)) w /= 2;
+    return w;
+  }
+  static constexpr int pick() {
+    int w = base();
+    if (var_wshift(V) < 0 && w > 1) w /= 2;
+    if (var_wshift(V) > 0 && 2 * w * T <= 1024 && lds_cap(2 * w) <= 163840) w *= 2;
     return w;
   }
   static constexpr int W = pick();
@@ -62,12 +75,40 @@ __device__ __forceinline__ T ld(const void* tab, int64_t i) {
 
 // Per-thread evaluator of an X factor for the elements of one transform along `axis`
 // (scalar members only: runtime-indexed arrays would live in scratch memory).
+// exp(-i v) (RE) or exp(-v) (IM) for an FP64 argument; in FP32 the argument is reduced
+// modulo 2π in FP64 before the single-precision sincos.
+template <typename T>
+__device__ __forceinline__ T expop(double v, int mode) {
+  using R = decltype(T{}.x);
+  if (mode == 1) return T{(R)exp(-v), R(0)};   // QSF_MODE_IM
+  if constexpr (sizeof(R) == 8) {
+    double sn, cs;
+    sincos(-v, &sn, &cs);
+    return T{cs, sn};
+  } else {
+    const double twopi = 6.283185307179586476925286766559;
+    const double r = v - twopi * rint(v * (1.0 / twopi));
+    float sn, cs;
+    __sincosf(-(float)r, &sn, &cs);
+    return T{cs, sn};
+  }
+}
+
+__device__ __forceinline__ double ee_closed(double x, double y, int model, double EE, double Esoft) {
+  if (model == 3) return EE / sqrt((x - y) * (x - y) + x * y + Esoft);  // EckhardSacha
+  return EE / sqrt((x - y) * (x - y) + Esoft);                          // Eberly
+}
+
 template <typename T, int N> struct XEval {
   T cst;
   const T* ax;
   const T* pr0;
   const T* pr1;
   int o0, o1, npr, kind;
+  // closed-form pair parameters (copied: keeping a pointer into the kernel argument would
+  // force the whole argument block into scratch memory)
+  double delta, EE, Esoft, xmin, dx;
+  int mode, model;
   __device__ void init(const XFactor& f, int axis, int i0, int i1, int i2) {
     using R = decltype(cst.x);
     auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
@@ -80,7 +121,32 @@ template <typename T, int N> struct XEval {
     o0 = o1 = 0;
     pr0 = pr1 = nullptr;
     kind = f.pair_kind;
-    if (kind != PAIR_NONE) {
+    delta = f.delta;
+    EE = f.EE;
+    Esoft = f.Esoft;
+    xmin = f.xmin;
+    dx = f.dx;
+    mode = f.mode;
+    model = f.model;
+    if (kind == PAIR_ONFLY) {
+      double sc = 0.0;   // pairs not involving the transform axis: one phase per thread
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (!(f.pair_mask & (1 << k))) continue;
+        const int p = k, q = (k + 1) % 3;
+        if (p != axis && q != axis) {
+          sc += ee_closed(xmin + id(p) * dx, xmin + id(q) * dx, model, EE, Esoft);
+          any = true;
+        } else {
+          const int o = (p == axis) ? id(q) : id(p);
+          if (npr == 0) o0 = o;
+          else o1 = o;
+          ++npr;
+        }
+      }
+      if (any) cst = cmul(cst, expop<T>(delta * sc, mode));
+    } else if (kind != PAIR_NONE) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (!(f.pair_mask & (1 << k))) continue;
@@ -98,13 +164,24 @@ template <typename T, int N> struct XEval {
       }
     }
   }
-  __device__ __forceinline__ T at(int n) const {
-    T r = cst;
-    if (ax) r = cmul(r, ax[n]);
-    if (npr > 0) r = cmul(r, kind == PAIR_DIFF ? pr0[abs(n - o0)] : pr0[n]);
+  // product of the pair terms that involve the transform axis (1 if none)
+  __device__ __forceinline__ T pair_at(int n) const {
+    using R = decltype(cst.x);
+    T r{R(1), R(0)};
+    if (kind == PAIR_ONFLY) {
+      if (npr == 0) return r;
+      const double xn = xmin + n * dx;
+      double sv = ee_closed(xn, xmin + o0 * dx, model, EE, Esoft);
+      if (npr > 1) sv += ee_closed(xn, xmin + o1 * dx, model, EE, Esoft);
+      return expop<T>(delta * sv, mode);
+    }
+    if (npr > 0) r = kind == PAIR_DIFF ? pr0[abs(n - o0)] : pr0[n];
     if (npr > 1) r = cmul(r, kind == PAIR_DIFF ? pr1[abs(n - o1)] : pr1[n]);
     return r;
   }
+  // factor without the axis-dependent pair part
+  __device__ __forceinline__ T base_at(int n) const { return ax ? cmul(cst, ax[n]) : cst; }
+  __device__ __forceinline__ T at(int n) const { return npr ? cmul(base_at(n), pair_at(n)) : base_at(n); }
 };
 
 __device__ __forceinline__ double pot_value(const ObsParams& o, const double* c, int D) {
@@ -141,13 +218,18 @@ __device__ __forceinline__ double pot_value(const ObsParams& o, const double* c,
   return v;
 }
 
-template <typename Real, int N, bool ROW>
-__global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
+// QSF_MIN_WAVES: minimum waves per SIMD the register allocation must allow.
+#ifndef QSF_MIN_WAVES
+#define QSF_MIN_WAVES 1
+#endif
+template <typename Real, int N, bool ROW, int V>
+__global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES)
     axis_pass_kernel(PassParams p) {
   using T = typename Tr<Real>::T;
-  using PL = PlanOf<N>;
+  constexpr int ALT = var_alt(V);
+  using PL = PlanOf<N, ALT>;
   constexpr int E = PL::E, TT = PL::T;
-  constexpr int W = Geo<Real, N, ROW>::W;
+  constexpr int W = Geo<Real, N, ROW, V>::W;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
 
@@ -206,10 +288,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
     constexpr int DIR = decltype(dir_tag)::value;
     if constexpr (ROW) {
       RowAddr<T> a{w * N};
-      fft<N, DIR>(v, t, lds, a, roots, p.tw_mode);
+      fft<N, ALT, DIR>(v, t, lds, a, roots, p.tw_mode);
     } else {
       ColAddr<W> a{w};
-      fft<N, DIR>(v, t, lds, a, roots, p.tw_mode);
+      fft<N, ALT, DIR>(v, t, lds, a, roots, p.tw_mode);
     }
   };
   using FWD = std::integral_constant<int, -1>;
@@ -217,113 +299,131 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
 
   if (p.flags & F_INV1) run_fft(INV{});
 
-  if (p.flags & F_XA) {
-    XEval<T, N> xe;
-    xe.init(p.xa, p.axis, i0, i1, i2);
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cmul(v[q], xe.at(t + TT * q));
-  }
-
-  if (p.flags & F_OBS) {
+  // X stage: A factor -> observables / region mask -> B factor, one loop over the elements.
+  // When both halves use the same pair-interaction table its lookup is shared.
+  const uint32_t XF = p.flags & (F_XA | F_OBS | F_MASK | F_XB);
+  if (XF) {
+    const bool doA = p.flags & F_XA, doB = p.flags & F_XB, obs = p.flags & F_OBS, msk = p.flags & F_MASK;
+    XEval<T, N> xa, xb;
+    if (doA) xa.init(p.xa, p.axis, i0, i1, i2);
+    if (doB) xb.init(p.xb, p.axis, i0, i1, i2);
+    const bool shared = doA && doB && xa.npr > 0 && p.xa.pair_kind == p.xb.pair_kind &&
+                        p.xa.pair == p.xb.pair && p.xa.pair_mask == p.xb.pair_mask &&
+                        p.xa.delta == p.xb.delta && p.xa.mode == p.xb.mode;
     const ObsParams& ob = p.obs;
-    double nrm = 0.0, ep = 0.0, ek = 0.0;
-    if (valid) {
-      const int D = p.n[2] > 1 ? 3 : (p.n[1] > 1 ? 2 : 1);
-      double kc = 0.0;
+    double nrm = 0.0, ep = 0.0, ek = 0.0, kc = 0.0;
+    const double* ka = nullptr;
+    int D = 1;
+    if (obs) {
+      D = p.n[2] > 1 ? 3 : (p.n[1] > 1 ? 2 : 1);
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         if (d != p.axis && ob.kin_w[d]) kc += ob.kin_w[d][id(d)];
-      const double* ka = p.axis == 0 ? ob.kin_w[0] : (p.axis == 1 ? ob.kin_w[1] : ob.kin_w[2]);
+      ka = p.axis == 0 ? ob.kin_w[0] : (p.axis == 1 ? ob.kin_w[1] : ob.kin_w[2]);
+    }
+    // region mask: offset of the other axes' nodes
+    int64_t mo = 0, msa = 0, na = 1;
+    if (msk) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (d == p.axis || p.mask_stride[d] == 0) continue;
+        const int64_t h = p.n[d] / 2, i = id(d);
+        const int64_t nd = i < h ? -i : i - p.n[d] + 1;
+        int64_t node = p.nCAP + nd;
+        node = node < 0 ? 0 : node;
+        mo += node * p.mask_stride[d];
+      }
+      msa = p.axis == 0 ? p.mask_stride[0] : (p.axis == 1 ? p.mask_stride[1] : p.mask_stride[2]);
+      na = p.axis == 0 ? p.n[0] : (p.axis == 1 ? p.n[1] : p.n[2]);
+    }
+    const bool gather = ROW && obs && ob.gather_rowptr;
+    if (!obs && !msk && !(ROW && p.er_out)) {
+      // common case (no observation this pass): compact loop, both halves per element
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        double a2 = (double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y;
+        const int ia = t + TT * q;
+        if (shared) {
+          const T pp = xa.pair_at(ia);
+          v[q] = cmul(v[q], cmul(cmul(xa.base_at(ia), xb.base_at(ia)), cmul(pp, pp)));
+        } else {
+          if (doA) v[q] = cmul(v[q], xa.at(ia));
+          if (doB) v[q] = cmul(v[q], xb.at(ia));
+        }
+      }
+    } else
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int ia = t + TT * q;
+      T pp;
+      if (shared) pp = xa.pair_at(ia);
+      if (doA) v[q] = cmul(v[q], shared ? cmul(xa.base_at(ia), pp) : xa.at(ia));
+      if (obs && valid) {
+        const double a2 = (double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y;
         nrm += a2;
         if (ob.want_epot) {
-          const int ia = t + TT * q;
           double co[3];
           co[0] = ob.xmin[0] + (p.axis == 0 ? ia : i0) * ob.dx[0];
           co[1] = ob.xmin[1] + (p.axis == 1 ? ia : i1) * ob.dx[1];
           co[2] = ob.xmin[2] + (p.axis == 2 ? ia : i2) * ob.dx[2];
           ep += a2 * pot_value(ob, co, D);
         }
-        if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[t + TT * q] : 0.0));
+        if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[ia] : 0.0));
+        if (ob.copy_out) reinterpret_cast<T*>(ob.copy_out)[base + (int64_t)ia * step] = v[q];
       }
-      if (ob.copy_out) {
-        T* cp = reinterpret_cast<T*>(ob.copy_out);
-#pragma unroll
-        for (int q = 0; q < E; ++q) cp[base + (int64_t)(t + TT * q) * step] = v[q];
-      }
-    }
-    // block reduction (fixed order: lanes, then waves)
-    for (int off = 32; off > 0; off >>= 1) {
-      nrm += __shfl_xor(nrm, off);
-      ep += __shfl_xor(ep, off);
-      ek += __shfl_xor(ek, off);
-    }
-    __syncthreads();
-    double* red = reinterpret_cast<double*>(smem_raw);
-    const int wave = tid >> 6, nwaves = (blockDim.x + 63) >> 6;
-    if ((tid & 63) == 0) { red[3 * wave] = nrm; red[3 * wave + 1] = ep; red[3 * wave + 2] = ek; }
-    __syncthreads();
-    if (tid == 0 && ob.partials) {
-      double a = 0.0, e2 = 0.0, e3 = 0.0;
-      for (int k = 0; k < nwaves; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
-      ob.partials[3 * blockIdx.x] = a;
-      ob.partials[3 * blockIdx.x + 1] = e2;
-      ob.partials[3 * blockIdx.x + 2] = e3;
-    }
-    __syncthreads();
-    if constexpr (ROW) {
-      if (ob.gather_rowptr) {
-        RowAddr<T> a{w * N};
-#pragma unroll
-        for (int q = 0; q < E; ++q) lds[a(t + TT * q)] = v[q];
-        __syncthreads();
-        if (valid) {
-          const int k0 = ob.gather_rowptr[o], k1 = ob.gather_rowptr[o + 1];
-          T* go = reinterpret_cast<T*>(ob.gather_out);
-          for (int k = k0 + t; k < k1; k += TT) go[k] = lds[a(ob.gather_pos[k])];
+      if (gather) lds[w * N + ia] = v[q];
+      if (ROW && p.er_out && valid) {
+        // multigrid source edges of the boundary state (2D main region, row i0 = o)
+        const int64_t nc = p.nCAP, n0 = p.n[0], n1 = N;
+        if (o < nc || o >= n0 - nc) {
+          const int64_t e = o < nc ? o : o - (n0 - 2 * nc);
+          reinterpret_cast<T*>(p.er_out)[e * n1 + ia] = v[q];
         }
-        __syncthreads();
+        if (ia < nc || ia >= n1 - nc) {
+          const int64_t e = ia < nc ? ia : ia - (n1 - 2 * nc);
+          reinterpret_cast<T*>(p.ec_out)[o * 2 * nc + e] = v[q];
+        }
       }
-    }
-  }
-
-  if (p.flags & F_MASK) {
-    // maskRegion (grid.h:582-597): one exact FP64 factor per point, from per-axis CAP nodes
-    int64_t mo = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (d == p.axis || p.mask_stride[d] == 0) continue;
-      const int64_t h = p.n[d] / 2, i = id(d);
-      const int64_t nd = i < h ? -i : i - p.n[d] + 1;
-      int64_t node = p.nCAP + nd;
-      node = node < 0 ? 0 : node;
-      mo += node * p.mask_stride[d];
-    }
-    const int64_t na = p.axis == 0 ? p.n[0] : (p.axis == 1 ? p.n[1] : p.n[2]);
-    const int64_t ha = na / 2;
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      int64_t off = mo;
-      const int64_t msa = p.axis == 0 ? p.mask_stride[0] : (p.axis == 1 ? p.mask_stride[1] : p.mask_stride[2]);
-      if (msa) {
-        const int64_t i = t + TT * q;
-        const int64_t nd = i < ha ? -i : i - na + 1;
-        int64_t node = p.nCAP + nd;
-        node = node < 0 ? 0 : node;
-        off += node * msa;
+      if (msk) {
+        // maskRegion (grid.h:582-597): one exact FP64 factor per point
+        int64_t off = mo;
+        if (msa) {
+          const int64_t nd = ia < na / 2 ? -ia : ia - na + 1;
+          int64_t node = p.nCAP + nd;
+          node = node < 0 ? 0 : node;
+          off += node * msa;
+        }
+        const double mk = p.mask_tab[off];
+        v[q] = T{(Real)((double)v[q].x * mk), (Real)((double)v[q].y * mk)};
       }
-      const double mk = p.mask_tab[off];
-      v[q] = T{(Real)((double)v[q].x * mk), (Real)((double)v[q].y * mk)};
+      if (doB) v[q] = cmul(v[q], shared ? cmul(xb.base_at(ia), pp) : xb.at(ia));
     }
-  }
-
-  if (p.flags & F_XB) {
-    XEval<T, N> xe;
-    xe.init(p.xb, p.axis, i0, i1, i2);
-#pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cmul(v[q], xe.at(t + TT * q));
+    if (obs) {
+      // block reduction (fixed order: lanes, then waves)
+      for (int off = 32; off > 0; off >>= 1) {
+        nrm += __shfl_xor(nrm, off);
+        ep += __shfl_xor(ep, off);
+        ek += __shfl_xor(ek, off);
+      }
+      __syncthreads();
+      if (gather && valid) {
+        const int k0 = ob.gather_rowptr[o], k1 = ob.gather_rowptr[o + 1];
+        T* go = reinterpret_cast<T*>(ob.gather_out);
+        for (int k = k0 + t; k < k1; k += TT) go[k] = lds[w * N + ob.gather_pos[k]];
+      }
+      __syncthreads();
+      double* red = reinterpret_cast<double*>(smem_raw);
+      const int wave = tid >> 6, nwaves = (blockDim.x + 63) >> 6;
+      if ((tid & 63) == 0) { red[3 * wave] = nrm; red[3 * wave + 1] = ep; red[3 * wave + 2] = ek; }
+      __syncthreads();
+      if (tid == 0 && ob.partials) {
+        double a = 0.0, e2 = 0.0, e3 = 0.0;
+        for (int k = 0; k < nwaves; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
+        ob.partials[3 * blockIdx.x] = a;
+        ob.partials[3 * blockIdx.x + 1] = e2;
+        ob.partials[3 * blockIdx.x + 2] = e3;
+      }
+      __syncthreads();
+    }
   }
 
   if (p.flags & F_FWD) run_fft(FWD{});
@@ -350,9 +450,9 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW>::THREADS))
 }
 
 
-template <typename Real, int N, bool ROW>
+template <typename Real, int N, bool ROW, int V>
 inline int launch_one(const PassParams& p, hipStream_t s) {
-  using G = Geo<Real, N, ROW>;
+  using G = Geo<Real, N, ROW, V>;
   int64_t tiles;
   if (ROW) tiles = (p.outer + G::W - 1) / G::W;
   else tiles = p.outer * ((p.inner + G::W - 1) / G::W);
@@ -360,44 +460,77 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
   if (lds < 16 * 3 * sizeof(double)) lds = 16 * 3 * sizeof(double);
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW>), dim3((unsigned)tiles), dim3(G::THREADS), lds, s, p);
+  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V>), dim3((unsigned)tiles), dim3(G::THREADS), lds, s, p);
   return (int)tiles;
 }
 
+template <typename Real, int N, bool ROW, int V>
+inline int64_t tiles_one(const PassParams& p) {
+  using G = Geo<Real, N, ROW, V>;
+  if (ROW) return (p.outer + G::W - 1) / G::W;
+  return p.outer * ((p.inner + G::W - 1) / G::W);
+}
+
+// Lengths with tuned variants; every other supported length has variant 0 only.
+inline bool has_variants(int64_t n) {
+  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 768 || n == 1536;
+}
+
+template <typename Real, bool ROW, int N>
+inline int launch_v(int v, const PassParams& p, hipStream_t s) {
+  if constexpr (N == 256 || N == 512 || N == 1024 || N == 2048 || N == 4096 || N == 768 || N == 1536) {
+    switch (v) {
+      case 1: return launch_one<Real, N, ROW, 1>(p, s);
+      case 2: return launch_one<Real, N, ROW, 2>(p, s);
+      case 3: return launch_one<Real, N, ROW, 3>(p, s);
+      case 4: return launch_one<Real, N, ROW, 4>(p, s);
+      case 5: return launch_one<Real, N, ROW, 5>(p, s);
+      default: break;
+    }
+  }
+  return launch_one<Real, N, ROW, 0>(p, s);
+}
+
+template <typename Real, bool ROW, int N>
+inline int64_t tiles_v(int v, const PassParams& p) {
+  if constexpr (N == 256 || N == 512 || N == 1024 || N == 2048 || N == 4096 || N == 768 || N == 1536) {
+    switch (v) {
+      case 1: return tiles_one<Real, N, ROW, 1>(p);
+      case 2: return tiles_one<Real, N, ROW, 2>(p);
+      case 3: return tiles_one<Real, N, ROW, 3>(p);
+      case 4: return tiles_one<Real, N, ROW, 4>(p);
+      case 5: return tiles_one<Real, N, ROW, 5>(p);
+      default: break;
+    }
+  }
+  return tiles_one<Real, N, ROW, 0>(p);
+}
+
+#define QSF_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) \
+  X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072)
 
 template <typename Real, bool ROW>
-inline int dispatch_n(int64_t n, const PassParams& p, hipStream_t s) {
+inline int dispatch_n(int64_t n, int v, const PassParams& p, hipStream_t s) {
   switch (n) {
-#define QSF_CASE(NN) case NN: return launch_one<Real, NN, ROW>(p, s);
-    QSF_CASE(8) QSF_CASE(16) QSF_CASE(32) QSF_CASE(64) QSF_CASE(128) QSF_CASE(256)
-    QSF_CASE(512) QSF_CASE(1024) QSF_CASE(2048) QSF_CASE(4096)
-    QSF_CASE(12) QSF_CASE(24) QSF_CASE(48) QSF_CASE(96) QSF_CASE(192) QSF_CASE(384)
-    QSF_CASE(768) QSF_CASE(1536) QSF_CASE(3072)
+#define QSF_CASE(NN) case NN: return launch_v<Real, ROW, NN>(v, p, s);
+    QSF_SIZES(QSF_CASE)
 #undef QSF_CASE
     default: return -1;
   }
 }
 
-
-// Number of blocks a pass launches (sizes the partial-sum buffers).
 template <typename Real, bool ROW>
-inline int64_t tiles_n(int64_t n, const PassParams& p) {
-  int W = 1;
+inline int64_t tiles_n(int64_t n, int v, const PassParams& p) {
   switch (n) {
-#define QSF_W(NN) case NN: W = Geo<Real, NN, ROW>::W; break;
-    QSF_W(8) QSF_W(16) QSF_W(32) QSF_W(64) QSF_W(128) QSF_W(256) QSF_W(512) QSF_W(1024)
-    QSF_W(2048) QSF_W(4096) QSF_W(12) QSF_W(24) QSF_W(48) QSF_W(96) QSF_W(192) QSF_W(384)
-    QSF_W(768) QSF_W(1536) QSF_W(3072)
-#undef QSF_W
+#define QSF_CASE(NN) case NN: return tiles_v<Real, ROW, NN>(v, p);
+    QSF_SIZES(QSF_CASE)
+#undef QSF_CASE
     default: return -1;
   }
-  if (ROW) return (p.outer + W - 1) / W;
-  return p.outer * ((p.inner + W - 1) / W);
 }
-
 
 }  // namespace qsf

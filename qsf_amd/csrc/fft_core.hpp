@@ -137,15 +137,23 @@ struct Plan {
   int r[8];
 };
 
-__host__ __device__ constexpr int plan_e(int n) {
-  if (n % 3 == 0) return n <= 12 ? n : (n <= 24 ? n : 24);
+// Elements per thread. `alt` selects the other register/LDS balance for the same length
+// (fewer elements per thread: more threads, more LDS exchanges, fewer VGPRs; or the reverse).
+__host__ __device__ constexpr int plan_e(int n, int alt = 0) {
+  if (n % 3 == 0) {
+    const int e = n <= 12 ? n : (n <= 24 ? n : 24);
+    return (alt && e == 24 && n >= 48) ? 12 : e;
+  }
   if (n <= 16) return n;
-  if (n <= 64 || n == 512) return 8;
-  return 16;
+  const int e = (n <= 64 || n == 512) ? 8 : 16;
+  if (!alt) return e;
+  if (e == 16 && n >= 128) return 8;
+  if (e == 8 && n >= 256) return 16;
+  return e;
 }
 
-__host__ __device__ constexpr Plan make_plan(int n) {
-  Plan p{n, plan_e(n), 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+__host__ __device__ constexpr Plan make_plan(int n, int alt = 0) {
+  Plan p{n, plan_e(n, alt), 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
   p.t = n / p.e;
   int rem = n;
   if (rem % 3 == 0) { p.r[p.nst++] = 3; rem /= 3; }
@@ -159,8 +167,8 @@ __host__ __device__ constexpr Plan make_plan(int n) {
   return p;
 }
 
-template <int N> struct PlanOf {
-  static constexpr Plan P = make_plan(N);
+template <int N, int ALT = 0> struct PlanOf {
+  static constexpr Plan P = make_plan(N, ALT);
   static constexpr int E = P.e;
   static constexpr int T = P.t;
   static constexpr int NST = P.nst;
@@ -175,10 +183,10 @@ template <int N> struct PlanOf {
 // ---------------------------------------------------------------- one Stockham transform
 // v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
 // thread's transform through addr(p). `roots[j] = exp(-2πi j/N)` (N entries).
-template <int N, int DIR, int S, typename T, typename Addr>
+template <int N, int ALT, int DIR, int S, typename T, typename Addr>
 __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                           int tw_mode) {
-  using PL = PlanOf<N>;
+  using PL = PlanOf<N, ALT>;
   constexpr int E = PL::E, TT = PL::T;
   constexpr int R = PL::template R<S>;
   constexpr int NS = PL::template NS<S>();
@@ -249,20 +257,20 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
   }
 }
 
-template <int N, int DIR, int S, typename T, typename Addr>
+template <int N, int ALT, int DIR, int S, typename T, typename Addr>
 __device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                            int tw_mode) {
-  if constexpr (S < PlanOf<N>::NST) {
-    fft_stage<N, DIR, S>(v, t, lds, addr, roots, tw_mode);
-    fft_stages<N, DIR, S + 1>(v, t, lds, addr, roots, tw_mode);
+  if constexpr (S < PlanOf<N, ALT>::NST) {
+    fft_stage<N, ALT, DIR, S>(v, t, lds, addr, roots, tw_mode);
+    fft_stages<N, ALT, DIR, S + 1>(v, t, lds, addr, roots, tw_mode);
   }
 }
 
 // tw_mode 0: all twiddles from the table; 1: two loads and products per butterfly.
-template <int N, int DIR, typename T, typename Addr>
+template <int N, int ALT, int DIR, typename T, typename Addr>
 __device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                     int tw_mode) {
-  fft_stages<N, DIR, 0>(v, t, lds, addr, roots, tw_mode);
+  fft_stages<N, ALT, DIR, 0>(v, t, lds, addr, roots, tw_mode);
 }
 
 }  // namespace qsf

@@ -20,7 +20,9 @@ enum PassFlags : uint32_t {
   F_NOSTORE = 1u << 9   // read-only pass (observables only)
 };
 
-enum PairKind : int32_t { PAIR_NONE = 0, PAIR_DIFF = 1, PAIR_FULL = 2 };
+// Pair interaction part of an X factor: none, a table indexed by |i-j| (Eberly), a full n*n
+// table, or evaluated per element from the closed form (argument in FP64, one sincos).
+enum PairKind : int32_t { PAIR_NONE = 0, PAIR_DIFF = 1, PAIR_FULL = 2, PAIR_ONFLY = 3 };
 
 // X factor = prod_a axis[a][i_a] * prod_{active pairs (p,q)} pair(i_p, i_q)
 struct XFactor {
@@ -29,6 +31,9 @@ struct XFactor {
   int32_t pair_kind;
   int32_t pair_mask;     // bit0: (0,1), bit1: (1,2), bit2: (2,0)
   int64_t pair_n;        // row length of a PAIR_FULL table
+  // PAIR_ONFLY: exp(-iδ·Σ ee) (RE) or exp(-δ·Σ ee) (IM) with ee of potential.h:130-136
+  double delta, EE, Esoft, xmin, dx;
+  int32_t mode, model;   // QSF_MODE_*, QSF_POT_EBERLY | QSF_POT_ECKHARDT_SACHA
 };
 
 // Observation of the step-boundary state (after the A factor).
@@ -67,6 +72,30 @@ struct PassParams {
   const double* mask_tab;
   int64_t mask_stride[3];  // 0 for free axes
   int64_t nCAP;
+  // 2D multigrid sources (written with the observation, before the mask): edge rows
+  // [2nCAP][n1] and edge columns [n0][2nCAP] of the step-boundary state
+  void* er_out;
+  void* ec_out;
+};
+
+// One transfer into a 2D receiving region along its freed axis (see transfer_pass.hip).
+struct TransferParams {
+  void* dst;                 // receiver state (n0, n1)
+  int64_t n0, n1;
+  int32_t axis;              // freed axis f: 0 = column pass, 1 = row pass
+  int32_t tw_mode;
+  const void* roots;         // roots of length n_f
+  int64_t nCAP;
+  const void* edge_in;       // source edge nodes along f
+  int32_t edge_in_layout;    // 0: [2nCAP][n_other] (edge rows), 1: [n_other][2nCAP] (edge columns)
+  int32_t edge_out_layout;   // layout of edge_out (0 rows, 1 columns)
+  void* edge_out;            // receiver edge nodes for the next group, or nullptr
+  const double* inv_tab;     // inv_mask(node) of the receiver
+  const double* corr_tab;    // group-1 mask_correction(node of the other axis), or nullptr
+  int32_t has_xa;            // apply the receiver's pending X factor before the add
+  XFactor xa;
+  const double* mask_tab;    // receiver region mask, or nullptr
+  int64_t mask_stride_axis, mask_stride_other;
 };
 
 }  // namespace qsf

@@ -8,9 +8,14 @@
 namespace qsf {
 
 bool axis_size_supported(int64_t n);
-// Launches one fused axis pass; returns the number of blocks (or -1 if n is unsupported).
-int launch_axis_pass(int precision, const PassParams& p, hipStream_t s);
-int64_t axis_pass_tiles(int precision, const PassParams& p);
+// Launches one fused axis pass with kernel variant v (0 = default; see axis_variants);
+// returns the number of blocks (or -1 if n is unsupported).
+int launch_axis_pass(int precision, const PassParams& p, hipStream_t s, int v = 0);
+int64_t axis_pass_tiles(int precision, const PassParams& p, int v = 0);
+// Number of compiled variants for an axis length (tile width x elements per thread).
+int axis_variants(int64_t n);
+// Fused 2D multigrid transfer into one receiving region (-1 if the length is unsupported).
+int launch_transfer(int precision, const TransferParams& p, hipStream_t s);
 
 // Per-axis phase tables of one substep, built on the device from FP64 scalars.
 //   X table: expOp(δ·(Ne(x_i)·use_ne + x_i·C)) · absorber(δ, i)
@@ -51,8 +56,10 @@ void launch_pair_table(int precision, const PairTableArgs& a, hipStream_t s);
 // roots[j] = exp(-2πi j/n), computed in FP64 and rounded to the precision.
 void launch_roots(int precision, void* out, int64_t n, hipStream_t s);
 
-// Sum of `count` blocks of `width` doubles into out[width] (fixed order).
-void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, hipStream_t s);
+// Sum of `count` blocks of `width` doubles into out[width] (fixed order). Large counts use a
+// second level through `scratch` (256*width doubles).
+void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, double* scratch,
+                            hipStream_t s);
 
 // Flux through border families from gathered boundary values.
 struct FluxArgs {

@@ -15,6 +15,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/qsf_hip.h"
@@ -74,6 +75,20 @@ struct qsf_ctx {
   struct Entry { std::string key; int64_t n; double ms, bytes; };
   std::vector<Entry> entries;
   int tw_mode[2] = {1, 0};  // FP64: two loads + products; FP32: table loads
+  bool pair_onfly = false;  // EckhardSacha pair phase per element instead of the n*n table
+  // kernel variant per (precision, row mode, axis length, grid size), chosen by the autotuner
+  struct TuneKey {
+    int prec, row;
+    int64_t n, m;
+    bool operator<(const TuneKey& o) const {
+      return std::tie(prec, row, n, m) < std::tie(o.prec, o.row, o.n, o.m);
+    }
+  };
+  std::map<TuneKey, std::pair<int, double>> tune;  // -> (variant, best ms)
+  int variant_of(int prec, int row, int64_t n, int64_t m) const {
+    auto it = tune.find({prec, row, n, m});
+    return it == tune.end() ? 0 : it->second.first;
+  }
   hipEvent_t event() {
     if (!spare.empty()) { hipEvent_t e = spare.back(); spare.pop_back(); return e; }
     hipEvent_t e;
@@ -119,7 +134,7 @@ struct Grid {
   bool bounded[3], isMain, many;
   std::vector<int> fftAxes;
   int pperm[3];
-  DBuf psi, aux, partials, red, kinw, masktab, invtab, corrtab, tabs;
+  DBuf psi, aux, partials, red, redscratch, kinw, masktab, invtab, corrtab, tabs;
   std::map<int64_t, std::unique_ptr<DBuf>> roots;
   std::map<std::pair<int64_t, int>, std::unique_ptr<DBuf>> pairs;
   std::unique_ptr<Gather> gather;
@@ -186,6 +201,7 @@ struct Grid {
     int64_t maxn = *std::max_element(n, n + 3);
     tabs.ensure((size_t)kSlots * 3 * maxn * cb);
     red.ensure(64 * sizeof(double));
+    redscratch.ensure(256 * 16 * sizeof(double));
     build_static_tables();
   }
 
@@ -339,11 +355,30 @@ struct Grid {
     for (int a = 0; a < D; ++a) f.axis[a] = args[a].out;
     f.pair_mask = pair_mask();
     if (f.pair_mask) {
-      f.pair = pair_table(delta, mode);
-      f.pair_kind = d.potential == QSF_POT_EBERLY ? PAIR_DIFF : PAIR_FULL;
       int a = 0;
       while (a < D && !bounded[a]) ++a;
       f.pair_n = n[a];
+      f.delta = delta;
+      f.mode = mode;
+      f.model = d.potential;
+      f.EE = d.Echarge * d.Echarge;
+      f.Esoft = d.Esoft;
+      f.xmin = xmin[a];
+      f.dx = dx[a];
+      if (d.potential == QSF_POT_EBERLY) {
+        // depends on |i-j| only: an n-entry table stays in L1/L2
+        f.pair = pair_table(delta, mode);
+        f.pair_kind = PAIR_DIFF;
+      } else if (ctx->pair_onfly) {
+        // EckhardSacha closed form per element (FP64 argument, one sincos)
+        f.pair = nullptr;
+        f.pair_kind = PAIR_ONFLY;
+      } else {
+        // EckhardSacha ee(x,y) is not a difference kernel: n*n phase table (measured faster
+        // than one FP64 sincos per element on MI355X)
+        f.pair = pair_table(delta, mode);
+        f.pair_kind = PAIR_FULL;
+      }
     }
     return f;
   }
@@ -405,11 +440,12 @@ struct Grid {
     pending_tables.clear();
   }
 
-  int run(PassParams& p) {
+  int run(PassParams& p, int force_variant = -1) {
     build_tables();
     p.tw_mode = ctx->tw_mode[prec];
+    const int var = force_variant >= 0 ? force_variant : ctx->variant_of(prec, p.inner == 1, n[p.axis], m);
     if (p.flags & F_OBS) {
-      int64_t tiles = axis_pass_tiles(prec, p);
+      int64_t tiles = axis_pass_tiles(prec, p, var);
       partials.ensure(tiles * 3 * sizeof(double));
       if (!p.obs.partials) p.obs.partials = partials.as<double>();
     }
@@ -419,19 +455,63 @@ struct Grid {
       eb = ctx->event();
       QSF_HIP(hipEventRecord(ea, st()));
     }
-    int k = launch_axis_pass(prec, p, st());
+    int k = launch_axis_pass(prec, p, st(), var);
     QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
     QSF_HIP(hipGetLastError());
     if (ctx->prof) {
       QSF_HIP(hipEventRecord(eb, st()));
       const bool row = p.inner == 1;
       std::string key = std::string("pass_") + (prec == QSF_FP64 ? "f64" : "f32") + (row ? "_row_n" : "_col_n") +
-                        std::to_string(n[p.axis]);
+                        std::to_string(n[p.axis]) + "_v" + std::to_string(var);
       const double bytes = (double)m * cb * ((p.flags & F_NOSTORE) ? 1.0 : 2.0);
       ctx->launches.push_back({key, ea, eb, bytes});
       if (ctx->launches.size() >= 4096) drain_prof(ctx);
     }
     return k;
+  }
+
+  // ---- autotuner: time every compiled variant of the passes this grid's step uses ----
+  void autotune(int reps) {
+    aux.ensure(m * cb);
+    QSF_HIP(hipMemcpyAsync(aux.p, psi.p, m * cb, hipMemcpyDeviceToDevice, st()));
+    const double C[3] = {0.01, 0.01, 0.01};
+    struct Cand { int axis; uint32_t flags; };
+    std::vector<Cand> cands;
+    if (fftAxes.size() >= 2) {
+      cands.push_back({D - 1, F_INV1 | F_XA | F_XB | F_FWD});
+      cands.push_back({0, F_FWD | F_P | F_INV2});
+      for (int a = 1; a <= D - 2; ++a) cands.push_back({a, F_FWD});
+    } else if (fftAxes.size() == 1) {
+      cands.push_back({fftAxes[0], F_XA | F_XB | F_FWD | F_P | F_INV2});
+    }
+    hipEvent_t e0, e1;
+    QSF_HIP(hipEventCreate(&e0));
+    QSF_HIP(hipEventCreate(&e1));
+    for (const Cand& c : cands) {
+      const int row = (c.axis == D - 1);
+      const int nv = axis_variants(n[c.axis]);
+      int best = 0;
+      double best_ms = 1e300;
+      for (int v = 0; v < nv; ++v) {
+        PassParams p = params(c.axis, aux.p, aux.p);
+        p.flags = c.flags;
+        if (c.flags & F_XA) p.xa = x_factor(QSF_MODE_RE, 0.01, C, true);
+        if (c.flags & F_XB) p.xb = x_factor(QSF_MODE_RE, 0.01, C, true);
+        if (c.flags & F_P) p_factor(QSF_MODE_RE, 0.01, C, true, true, p.pf);
+        run(p, v);  // warm-up (and table build) outside the timed region
+        QSF_HIP(hipEventRecord(e0, st()));
+        for (int r = 0; r < reps; ++r) run(p, v);
+        QSF_HIP(hipEventRecord(e1, st()));
+        QSF_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        QSF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / reps;
+        if (per < best_ms) { best_ms = per; best = v; }
+      }
+      ctx->tune[{prec, row, n[c.axis], m}] = {best, best_ms};
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
 
   // ---- standalone members ----
@@ -489,7 +569,7 @@ struct Grid {
       for (int a = 0; a < D; ++a) p.obs.kin_w[a] = kinw.as<double>() + a * maxn;
     }
     int tiles = run(p);
-    launch_reduce_partials(partials.as<double>(), tiles, 3, red.as<double>(), st());
+    launch_reduce_partials(partials.as<double>(), tiles, 3, red.as<double>(), redscratch.as<double>(), st());
     QSF_HIP(hipMemcpyAsync(host3, red.p, 3 * sizeof(double), hipMemcpyDeviceToHost, st()));
     QSF_HIP(hipStreamSynchronize(st()));
   }
@@ -630,7 +710,7 @@ struct Grid {
     up(g->pts, pt.data(), pt.size() * 4);
     up(g->bvec, bv.data(), bv.size() * 8);
     g->values.ensure(std::max<int64_t>(g->nvals, 1) * cb);
-    g->partials.ensure(64 * 16 * sizeof(double));
+    g->partials.ensure(1024 * 16 * sizeof(double));
     g->out.ensure(16 * sizeof(double));
     gather = std::move(g);
   }
@@ -762,6 +842,64 @@ struct Set {
         }
   }
 
+  // Edge buffers of the fused transfer: main region edge rows/columns, region X edge columns
+  // and region Y edge rows (the sources of the XY region).
+  DBuf er_main, ec_main, ec_x, er_y;
+  void ensure_edges() {
+    const Grid& m0 = *g[0];
+    const size_t bytes_r = 2 * m0.d.nCAP * m0.n[1] * m0.cb, bytes_c = m0.n[0] * 2 * m0.d.nCAP * m0.cb;
+    er_main.ensure(bytes_r);
+    ec_main.ensure(bytes_c);
+    ec_x.ensure(bytes_c);
+    er_y.ensure(bytes_r);
+  }
+
+  // Fused transfer() + maskRegion() of the receiving regions, reading the main region's edges
+  // written by its last observation pass. `xa[r]` is region r's pending X factor (or nullptr).
+  void receive_fused(const XFactor* const* xa) {
+    struct R { int region, axis; const DBuf* in; int in_layout; DBuf* out; int out_layout; bool mask; };
+    const R plan[4] = {
+        {1, 0, &er_main, 0, &ec_x, 1, true},    // X  <- NO along X
+        {2, 1, &ec_main, 1, &er_y, 0, true},    // Y  <- NO along Y
+        {3, 1, &ec_x, 1, nullptr, 0, false},    // XY <- X  along Y (source 0)
+        {3, 0, &er_y, 0, nullptr, 0, false},    // XY <- Y  along X (source 1)
+    };
+    bool xa_used[4] = {false, false, false, false};
+    for (const R& r : plan) {
+      Grid& rg = *g[r.region];
+      rg.build_tables();
+      TransferParams tp{};
+      tp.dst = rg.psi.p;
+      tp.n0 = rg.n[0];
+      tp.n1 = rg.n[1];
+      tp.axis = r.axis;
+      tp.tw_mode = ctx->tw_mode[rg.prec];
+      tp.roots = rg.root(rg.n[r.axis]);
+      tp.nCAP = rg.d.nCAP;
+      tp.edge_in = r.in->p;
+      tp.edge_in_layout = r.in_layout;
+      tp.edge_out = r.out ? r.out->p : nullptr;
+      tp.edge_out_layout = r.out_layout;
+      tp.inv_tab = rg.invtab.as<double>();
+      tp.corr_tab = rg.group == 1 ? rg.corrtab.as<double>() : nullptr;
+      if (xa[r.region] && !xa_used[r.region]) {
+        tp.has_xa = 1;
+        tp.xa = *xa[r.region];
+        xa_used[r.region] = true;
+      }
+      if (r.mask) {
+        const int64_t c = rg.d.nCAP + 1;
+        const int64_t st1 = rg.bounded[1] ? 1 : 0, st0 = rg.bounded[0] ? c : 0;
+        tp.mask_tab = rg.masktab.as<double>();
+        tp.mask_stride_axis = r.axis == 0 ? st0 : st1;
+        tp.mask_stride_other = r.axis == 0 ? st1 : st0;
+      }
+      const int k = launch_transfer(rg.prec, tp, ctx->stream);
+      QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no transfer kernel for this axis length");
+      QSF_HIP(hipGetLastError());
+    }
+  }
+
   // transfer() in group order; every source must be in X along the freed axis.
   void transfer() {
     for (int gI = 1; gI < groupCount; ++gI)
@@ -851,6 +989,7 @@ struct Prop {
   std::vector<bool> deferred;     // state is "after a P pass": pending inverse (main) + pending X
   std::vector<PendingX> pend;
   bool pend_obs = false;
+  bool pend_post = false;   // multigrid transfer + masks of the last step boundary not yet run
   double pend_obs_C[3] = {0, 0, 0};
   int pend_row = -1;
   bool pend_ekin = false;
@@ -919,9 +1058,25 @@ struct Prop {
   // Each P substep is one row pass on the last axis (inverse of the previous P, the X factors,
   // forward FFT) and column passes for the other axes; the trailing X substep of the step
   // stays pending and is applied by the next pass that reads the grid.
+  // The main region's first pass of a step observes the previous boundary state and, for a
+  // multigrid, writes the transfer edges and applies its region mask; the receiving regions
+  // then run their fused transfers before their own substeps.
+  void run_receives() {
+    std::vector<const XFactor*> xa(set->g.size(), nullptr);
+    for (size_t r = 1; r < set->g.size(); ++r)
+      if (deferred[r] && pend[r].on) xa[r] = &pend[r].f;
+    set->receive_fused(xa.data());
+    for (size_t r = 1; r < set->g.size(); ++r) {
+      deferred[r] = false;
+      pend[r].on = false;
+    }
+    pend_post = false;
+  }
+
   void exec_fused(const std::vector<SubOp>& ops) {
     for (size_t r = 0; r < set->g.size(); ++r) {
       Grid& g = *set->g[r];
+      if (r == 1 && pend_post) run_receives();
       const bool obs_here = (r == 0) && pend_obs;
       for (size_t SI = 0; SI < ops.size(); ++SI) {
         const SubOp& o = ops[SI];
@@ -931,7 +1086,7 @@ struct Prop {
         const void* pf[3];
         g.p_factor(pd.mode, o.delta, o.C, couples() == QSF_REP_P, true, pf);
         const bool first = (SI == 1);
-        run_p_substep(g, r, fb, pf, first && obs_here);
+        run_p_substep(g, r, fb, pf, first && obs_here, first && r == 0 && pend_post);
       }
       const SubOp& xl = ops.back();
       pend[r].on = true;
@@ -939,6 +1094,18 @@ struct Prop {
       deferred[r] = true;
     }
     if (pend_obs) pend_obs = false;
+    if (set->multi) pend_post = true;
+  }
+
+  bool fused_multi() const { return set->multi && set->D == 2; }
+
+  // multigrid post-compute work of the main region folded into its pass
+  void add_post(Grid& g, PassParams& p) {
+    set->ensure_edges();
+    p.er_out = set->er_main.p;
+    p.ec_out = set->ec_main.p;
+    p.flags |= F_MASK;
+    g.fill_mask(p);
   }
 
   void obs_into(Grid& g, PassParams& p) {
@@ -954,7 +1121,7 @@ struct Prop {
   void after_obs(Grid& g, int tiles) {
     // reduce [norm, epot] of the boundary state into this row's device record
     double* rec = devrec.as<double>() + (size_t)pend_row * (3 + 16);
-    launch_reduce_partials(g.partials.as<double>(), tiles, 3, rec, st());
+    launch_reduce_partials(g.partials.as<double>(), tiles, 3, rec, g.redscratch.as<double>(), st());
     if (pd.nborders > 0 && (pd.outputs & QSF_OUT_FLUX))
       g.flux_from_gather(couples() == QSF_REP_P ? pend_obs_C : nullptr, rec + 3, fluxint.as<double>(), dt);
     if (pend_ekin) {
@@ -972,12 +1139,12 @@ struct Prop {
       for (int a = 0; a < gg.D; ++a) q.obs.kin_w[a] = gg.kinw.as<double>() + a * maxn;
       int t2 = gg.run(q);
       DBuf& tmp = gg.red;
-      launch_reduce_partials(gg.partials.as<double>(), t2, 3, tmp.as<double>(), st());
+      launch_reduce_partials(gg.partials.as<double>(), t2, 3, tmp.as<double>(), gg.redscratch.as<double>(), st());
       QSF_HIP(hipMemcpyAsync(rec + 2, tmp.as<double>() + 2, sizeof(double), hipMemcpyDeviceToDevice, st()));
     }
   }
 
-  void run_p_substep(Grid& g, size_t r, const XFactor& fb, const void* const* pf, bool observe) {
+  void run_p_substep(Grid& g, size_t r, const XFactor& fb, const void* const* pf, bool observe, bool post) {
     const int D = g.D;
     const auto& ax = g.fftAxes;
     const bool defer = deferred[r];
@@ -1014,6 +1181,7 @@ struct Prop {
       if (defer) {
         p.flags |= F_INV1;
         if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+        if (post) add_post(g, p);
       }
       if (observe) obs_into(g, p);
       int tiles = g.run(p);
@@ -1041,7 +1209,12 @@ struct Prop {
 
   // Apply whatever the pipeline left pending so every grid holds its step-boundary state.
   void flush() {
+    const bool post = pend_post;
     for (size_t r = 0; r < set->g.size(); ++r) {
+      if (r == 1 && post) {
+        run_receives();
+        break;  // the receives applied every region's pending X factor
+      }
       if (!deferred[r]) continue;
       Grid& g = *set->g[r];
       const bool observe = (r == 0) && pend_obs;
@@ -1050,6 +1223,7 @@ struct Prop {
       p.flags = 0;
       if (ax.size() >= 2) p.flags |= F_INV1;
       if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+      if (r == 0 && post) add_post(g, p);
       if (observe) obs_into(g, p);
       if (p.flags) {
         int tiles = g.run(p);
@@ -1059,6 +1233,7 @@ struct Prop {
       pend[r].on = false;
     }
     pend_obs = false;
+    pend_post = false;
   }
 
   // ---- computeEach (propagator.h:275-300), canonical output order of qsf_types.h ----
@@ -1164,6 +1339,7 @@ struct Prop {
 
   void post_compute() {
     if (!set->multi) return;
+    if (fused() && fused_multi()) return;  // runs inside the next pass (pend_post)
     if (fused()) flush();
     set->post_compute();
   }
@@ -1222,6 +1398,7 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
   auto* c = new qsf_ctx();
   c->device = device;
   if (const char* e = std::getenv("QSF_TW_MODE")) c->tw_mode[0] = c->tw_mode[1] = std::atoi(e);
+  if (const char* e = std::getenv("QSF_PAIR_ONFLY")) c->pair_onfly = std::atoi(e) != 0;
   int rc = guard(nullptr, [&] {
     int count = 0;
     QSF_HIP(hipGetDeviceCount(&count));
@@ -1268,6 +1445,36 @@ int qsf_prof_entry(qsf_ctx* c, int i, char* name, int len, int64_t* k, double* m
   if (k) *k = e.n;
   if (ms) *ms = e.ms;
   if (bytes) *bytes = e.bytes;
+  return QSF_OK;
+}
+
+int qsf_autotune(qsf_grid* g, int reps) {
+  if (!g || reps < 1) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] { g->g->autotune(reps); });
+}
+
+int qsf_tune_count(qsf_ctx* c, int* n) {
+  if (!c || !n) return QSF_ERR_ARG;
+  *n = (int)c->tune.size();
+  return QSF_OK;
+}
+
+int qsf_tune_entry(qsf_ctx* c, int i, int* prec, int* row, int64_t* n, int64_t* m, int* variant, double* ms) {
+  if (!c || i < 0 || i >= (int)c->tune.size()) return QSF_ERR_ARG;
+  auto it = c->tune.begin();
+  std::advance(it, i);
+  if (prec) *prec = it->first.prec;
+  if (row) *row = it->first.row;
+  if (n) *n = it->first.n;
+  if (m) *m = it->first.m;
+  if (variant) *variant = it->second.first;
+  if (ms) *ms = it->second.second;
+  return QSF_OK;
+}
+
+int qsf_tune_set(qsf_ctx* c, int prec, int row, int64_t n, int64_t m, int variant) {
+  if (!c || variant < 0 || variant >= qsf::axis_variants(n)) return QSF_ERR_ARG;
+  c->tune[{prec, row, n, m}] = {variant, 0.0};
   return QSF_OK;
 }
 

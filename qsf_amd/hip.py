@@ -13,7 +13,7 @@ import numpy as np
 from . import qsf_types as T
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqsf_hip.so")
+LIB_PATH = os.environ.get("QSF_HIP_LIB") or os.path.join(_HERE, "libqsf_hip.so")
 _LIB = None
 
 # (restype, argtypes) of every exported entry point
@@ -60,6 +60,11 @@ SIGNATURES = {
     "qsf_prof_begin": (_I, [_P]),
     "qsf_prof_end": (_I, [_P, C.POINTER(_I)]),
     "qsf_prof_entry": (_I, [_P, _I, C.c_char_p, _I, C.POINTER(_I64), _dp, _dp]),
+    "qsf_autotune": (_I, [_P, _I]),
+    "qsf_tune_count": (_I, [_P, C.POINTER(_I)]),
+    "qsf_tune_entry": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I64), C.POINTER(_I64),
+                            C.POINTER(_I), _dp]),
+    "qsf_tune_set": (_I, [_P, _I, _I, _I64, _I64, _I]),
 }
 
 
@@ -137,6 +142,22 @@ class Context:
             out[buf.value.decode()] = (k.value, ms.value, by.value)
         return out
 
+    def tune_table(self):
+        """[(precision, row, n, grid_size, variant, best_ms)] chosen by the autotuner."""
+        n = C.c_int()
+        self.check(load().qsf_tune_count(self.h, C.byref(n)), "tune_count")
+        out = []
+        for i in range(n.value):
+            pr, row, v = C.c_int(), C.c_int(), C.c_int()
+            nn, m, ms = C.c_int64(), C.c_int64(), C.c_double()
+            self.check(load().qsf_tune_entry(self.h, i, C.byref(pr), C.byref(row), C.byref(nn), C.byref(m),
+                                             C.byref(v), C.byref(ms)), "tune_entry")
+            out.append((pr.value, row.value, nn.value, m.value, v.value, ms.value))
+        return out
+
+    def tune_set(self, precision, row, n, grid_size, variant):
+        self.check(load().qsf_tune_set(self.h, precision, row, n, grid_size, variant), "tune_set")
+
     def elapsed_ms(self):
         v = C.c_double()
         self.check(load().qsf_timer_elapsed_ms(self.h, C.byref(v)), "timer_elapsed")
@@ -156,6 +177,37 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def tuning_cache_path():
+    base = os.environ.get("QSF_TUNE_CACHE") or os.path.join(
+        os.environ.get("XDG_CACHE_HOME", os.path.expanduser("~/.cache")), "qsf_amd", "autotune.json")
+    return base
+
+
+def load_tuning(ctx, path=None):
+    """Apply kernel-variant choices saved by save_tuning (missing file: nothing to do)."""
+    import json
+    path = path or tuning_cache_path()
+    try:
+        with open(path) as f:
+            entries = json.load(f)
+    except (OSError, ValueError):
+        return 0
+    for e in entries:
+        ctx.tune_set(e["precision"], e["row"], e["n"], e["grid_size"], e["variant"])
+    return len(entries)
+
+
+def save_tuning(ctx, path=None):
+    import json
+    path = path or tuning_cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    rows = [dict(precision=p, row=r, n=n, grid_size=m, variant=v, best_ms=ms)
+            for (p, r, n, m, v, ms) in ctx.tune_table()]
+    with open(path, "w") as f:
+        json.dump(rows, f, indent=1)
+    return path
 
 
 class Grid:
@@ -217,6 +269,10 @@ class Grid:
 
     def mask_region(self):
         self.ctx.check(load().qsf_mask_region(self.h), "mask_region")
+
+    def autotune(self, reps=5):
+        """Pick the fastest compiled kernel variant of each pass this grid's step uses."""
+        self.ctx.check(load().qsf_autotune(self.h, reps), "autotune")
 
 
 class GridSet:
