@@ -114,6 +114,25 @@ int qsf_prop_flux_integral(qsf_prop* p, double* out);
 /* 0 = fused axis passes (default), 1 = one standalone call per reference member. */
 int qsf_prop_set_exec(qsf_prop* p, int exec_mode);
 
+/* ---- batches: many single-region 2D solves (real-time Strang, velocity gauge, one field each)
+ *      advanced together. One step = one batched launch per (size group, pass) driven by a
+ *      device step counter; with use_graph it is captured once and replayed as a hipGraph.
+ *      Each solve follows its own SplitPropagator loop (max_steps+2 steps, propagator.h:153,
+ *      443-451); finished solves drop out of the launches. ---- */
+typedef struct qsf_batch qsf_batch;
+int qsf_batch_create(qsf_ctx* ctx, const qsf_grid_desc* grids, const qsf_field_desc* fields, int nsolves,
+                     const qsf_prop_desc* desc, qsf_batch** out);
+int qsf_batch_destroy(qsf_batch* b);
+int qsf_batch_upload(qsf_batch* b, int solve, const double* interleaved);
+int qsf_batch_download(qsf_batch* b, int solve, double* interleaved);
+int qsf_batch_start(qsf_batch* b);                                  /* computeEach<AT_START> */
+int qsf_batch_run(qsf_batch* b, int64_t steps, int use_graph, int* all_done);
+int qsf_batch_finish(qsf_batch* b);                                 /* end states + AT_END rows */
+int qsf_batch_info(qsf_batch* b, int solve, int64_t* max_steps, int64_t* grid_size);
+/* rows: step, time, norm, flux[nb] per observed step boundary */
+int qsf_batch_records(qsf_batch* b, int solve, double* out, int max_rows, int* rows);
+int qsf_batch_flux_integral(qsf_batch* b, int solve, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -329,3 +329,106 @@ void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_
 }
 
 }  // namespace qsf
+
+namespace qsf {
+
+template <typename Real>
+__global__ void batch_ptables_kernel(const BatchTableArgs* args, const int64_t* step_ctr) {
+  using T = typename cvec<Real>::T;
+  const BatchTableArgs& a = args[blockIdx.y];
+  const int64_t k = *step_ctr;
+  if (k >= a.total_steps) return;
+  const int ax = blockIdx.z;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n[ax]) return;
+  const double C = a.coupling[3 * k + a.storage_axis[ax]];
+  const double p = a.dp[ax] * (double)(i < a.pn2[ax] ? i : i - a.pn[ax]);
+  const double v = a.delta * ((0.5 * p * p) + (p * C));
+  double sn, cs;
+  sincos(-v, &sn, &cs);
+  const double nrm = 1.0 / (double)a.n[ax];
+  reinterpret_cast<T*>(a.out[ax])[i] = T{(Real)(cs * nrm), (Real)(sn * nrm)};
+}
+
+void launch_batch_ptables(int precision, const BatchTableArgs* dev_args, int nsolves, int64_t nmax,
+                          const int64_t* step_ctr, hipStream_t s) {
+  dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)nsolves, 2);
+  if (precision == QSF_FP64) hipLaunchKernelGGL(batch_ptables_kernel<double>, grid, dim3(256), 0, s, dev_args, step_ctr);
+  else hipLaunchKernelGGL(batch_ptables_kernel<float>, grid, dim3(256), 0, s, dev_args, step_ctr);
+}
+
+// One block per solve: norm from the pass partials and the flux families from the gathered
+// boundary points (same fixed-order sums as the single-solve path).
+template <typename Real>
+__global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, const int64_t* step_ctr) {
+  using T = typename cvec<Real>::T;
+  const BatchObsArgs& a = args[blockIdx.x];
+  const int64_t k = *step_ctr;            // steps completed at the observed boundary
+  if (k > a.total_steps || k < 1) return;
+  __shared__ double red[256];
+  double* row = a.rec + k * (1 + geo.nb);
+  double acc = 0.0;
+  for (int64_t b = threadIdx.x; b < a.tiles; b += 256) acc += a.partials[3 * b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) row[0] = red[0] * a.dV;
+  __syncthreads();
+  const double* C = a.coupling + 3 * (k - 1);   // A after the step's P substep
+  const T* g = reinterpret_cast<const T*>(a.gathered);
+  const int D = geo.D, nb = geo.nb, stride = 1 + 2 * D;
+  for (int f = 0; f < nb; ++f) {
+    double s = 0.0;
+    for (int64_t pidx = threadIdx.x; pidx < geo.npts; pidx += 256) {
+      const int32_t* q = geo.pts + pidx * stride;
+      const T c = g[q[0]];
+      const double cr = c.x, ci = c.y, nrm = cr * cr + ci * ci;
+      double j = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double bv = geo.bvec[(pidx * nb + f) * D + d];
+        if (bv == 0.0) continue;
+        double jd = 0.0;
+        const int32_t sp = q[1 + d], sm = q[1 + D + d];
+        if (sp >= 0 && sm >= 0) {
+          const double dr = (double)g[sp].x - (double)g[sm].x, di = (double)g[sp].y - (double)g[sm].y;
+          jd = geo.inv_2dx[d] * (cr * di - ci * dr);
+        }
+        jd += C[d] * nrm;
+        j += jd * bv;
+      }
+      s += j;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int t = 128; t > 0; t >>= 1) {
+      if ((int)threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      row[1 + f] = red[0] * geo.scale;
+      a.flux_int[f] += row[1 + f] * geo.dt;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_batch_obs(const BatchObsArgs* dev_args, int nsolves, const BatchFluxGeo& geo,
+                      const int64_t* step_ctr, hipStream_t s) {
+  if (geo.precision == QSF_FP64)
+    hipLaunchKernelGGL(batch_obs_kernel<double>, dim3(nsolves), dim3(256), 0, s, dev_args, geo, step_ctr);
+  else
+    hipLaunchKernelGGL(batch_obs_kernel<float>, dim3(nsolves), dim3(256), 0, s, dev_args, geo, step_ctr);
+}
+
+__global__ void step_increment_kernel(int64_t* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c += 1;
+}
+
+void launch_step_increment(int64_t* step_ctr, hipStream_t s) {
+  hipLaunchKernelGGL(step_increment_kernel, dim3(1), dim3(64), 0, s, step_ctr);
+}
+
+}  // namespace qsf

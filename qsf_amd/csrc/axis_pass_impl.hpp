@@ -275,6 +275,23 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
   const T* roots = reinterpret_cast<const T*>(p.roots);
   const T* src = reinterpret_cast<const T*>(p.src);
   T* dst = reinterpret_cast<T*>(p.dst);
+  const void* pf0 = p.pf[0];
+  const void* pf1 = p.pf[1];
+  const void* pf2 = p.pf[2];
+  double* partials = p.obs.partials;
+  void* gather_out = p.obs.gather_out;
+  if (p.batch) {
+    // batched launch: this block's solve (whole block leaves together, before any barrier)
+    const BatchEntry& be = p.batch[blockIdx.y];
+    if (*p.step_ctr >= be.total_steps) return;
+    src = reinterpret_cast<const T*>(be.psi);
+    dst = reinterpret_cast<T*>(be.psi);
+    pf0 = be.pf[0];
+    pf1 = be.pf[1];
+    pf2 = be.pf[2];
+    partials = be.partials;
+    gather_out = be.gather_out;
+  }
 
   T v[E];
 #pragma unroll
@@ -407,7 +424,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
       __syncthreads();
       if (gather && valid) {
         const int k0 = ob.gather_rowptr[o], k1 = ob.gather_rowptr[o + 1];
-        T* go = reinterpret_cast<T*>(ob.gather_out);
+        T* go = reinterpret_cast<T*>(gather_out);
         for (int k = k0 + t; k < k1; k += TT) go[k] = lds[w * N + ob.gather_pos[k]];
       }
       __syncthreads();
@@ -415,12 +432,12 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
       const int wave = tid >> 6, nwaves = (blockDim.x + 63) >> 6;
       if ((tid & 63) == 0) { red[3 * wave] = nrm; red[3 * wave + 1] = ep; red[3 * wave + 2] = ek; }
       __syncthreads();
-      if (tid == 0 && ob.partials) {
+      if (tid == 0 && partials) {
         double a = 0.0, e2 = 0.0, e3 = 0.0;
         for (int k = 0; k < nwaves; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
-        ob.partials[3 * blockIdx.x] = a;
-        ob.partials[3 * blockIdx.x + 1] = e2;
-        ob.partials[3 * blockIdx.x + 2] = e3;
+        partials[3 * blockIdx.x] = a;
+        partials[3 * blockIdx.x + 1] = e2;
+        partials[3 * blockIdx.x + 2] = e3;
       }
       __syncthreads();
     }
@@ -430,10 +447,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
 
   if (p.flags & F_P) {
     T cst{Real(1), Real(0)};
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      if (d != p.axis && p.pf[d]) cst = cmul(cst, ld<T>(p.pf[d], id(d)));
-    const T* pa = reinterpret_cast<const T*>(p.axis == 0 ? p.pf[0] : (p.axis == 1 ? p.pf[1] : p.pf[2]));
+    if (p.axis != 0 && pf0) cst = cmul(cst, ld<T>(pf0, i0));
+    if (p.axis != 1 && pf1) cst = cmul(cst, ld<T>(pf1, i1));
+    if (p.axis != 2 && pf2) cst = cmul(cst, ld<T>(pf2, i2));
+    const T* pa = reinterpret_cast<const T*>(p.axis == 0 ? pf0 : (p.axis == 1 ? pf1 : pf2));
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       T f = pa ? cmul(cst, pa[t + TT * q]) : cst;
@@ -464,7 +481,8 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
-  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V>), dim3((unsigned)tiles), dim3(G::THREADS), lds, s, p);
+  const unsigned ny = p.batch ? (unsigned)p.nbatch : 1u;
+  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V>), dim3((unsigned)tiles, ny), dim3(G::THREADS), lds, s, p);
   return (int)tiles;
 }
 

@@ -53,7 +53,19 @@ struct ObsParams {
   void* copy_out;
 };
 
+// One solve of a batched pass (blockIdx.y selects the entry).
+struct BatchEntry {
+  void* psi;               // the solve's state (same shape for every entry of a launch)
+  const void* pf[3];       // its P factor tables of the current step
+  double* partials;        // its per-block observable sums
+  void* gather_out;        // its gathered flux points
+  int64_t total_steps;     // steps the solve takes; later steps of the batch skip it
+};
+
 struct PassParams {
+  const BatchEntry* batch; // non-null: batched launch, per-solve pointers from batch[blockIdx.y]
+  const int64_t* step_ctr; // device step counter of a batched launch
+  int32_t nbatch;          // number of solves (gridDim.y) of a batched launch
   const void* src;
   void* dst;
   int64_t n[3];          // padded 3D shape (n0, n1, n2)

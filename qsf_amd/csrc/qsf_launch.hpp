@@ -100,3 +100,40 @@ struct SliceArgs {
 void launch_slice_2d(int precision, const SliceArgs& a, hipStream_t s);
 void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s);
 }  // namespace qsf
+
+namespace qsf {
+// ---- batched solves (one launch covers every solve of a size group; blockIdx.y = solve) ----
+struct BatchTableArgs {       // P tables of one solve for the current device step
+  void* out[2];
+  int64_t n[2], pn[2], pn2[2];
+  double dp[2];
+  double delta;               // dt of the P substep
+  const double* coupling;     // [total_steps][3]: A per axis of each step's P substep
+  int64_t total_steps;
+  int32_t storage_axis[2];    // coupling / P coordinates of the reference's storage axis
+};
+void launch_batch_ptables(int precision, const BatchTableArgs* dev_args, int nsolves, int64_t nmax,
+                          const int64_t* step_ctr, hipStream_t s);
+
+struct BatchObsArgs {         // observables of one solve at the boundary the pass observed
+  const double* partials;
+  int64_t tiles;
+  const void* gathered;
+  const double* coupling;     // [total_steps][3]
+  int64_t total_steps;
+  double* rec;                // [total_steps + 1][1 + nb]: norm, flux[nb]
+  double* flux_int;           // [nb]
+  double dV;
+};
+struct BatchFluxGeo {
+  const int32_t* pts;
+  const double* bvec;
+  int64_t npts;
+  int32_t D, nb, precision;
+  double inv_2dx[3], scale, dt;
+};
+// Observes the boundary after step *step_ctr (called before the counter advances).
+void launch_batch_obs(const BatchObsArgs* dev_args, int nsolves, const BatchFluxGeo& geo,
+                      const int64_t* step_ctr, hipStream_t s);
+void launch_step_increment(int64_t* step_ctr, hipStream_t s);
+}  // namespace qsf

@@ -46,6 +46,20 @@ struct DBuf {
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DBuf() {
     if (p) (void)hipFree(p);
   }
@@ -1356,6 +1370,268 @@ struct Prop {
   }
 };
 
+// ----------------------------------------------------------------------------------------
+// Batches: many single-region 2D solves (Strang, velocity gauge) advanced together. Solves are
+// grouped by grid description; one step is a fixed sequence of batched launches driven by a
+// device step counter, so it is captured once into a hipGraph and replayed.
+// ----------------------------------------------------------------------------------------
+struct Batch {
+  qsf_ctx* ctx;
+  qsf_prop_desc pd;
+  int prec = QSF_FP64;
+  struct Solve {
+    std::unique_ptr<Grid> g;
+    int64_t max_steps = 0, total = 0;
+    std::vector<double> times;      // timer after each step (host schedule)
+    std::vector<double> coup;       // [total][3]
+    DBuf coupling, rec, fluxint, partials, ptab, gathered;
+    std::vector<double> start_row, end_row;
+    int group = 0;
+  };
+  std::vector<Solve> solves;
+  struct Group {
+    std::vector<int> members;
+    Grid* lead = nullptr;
+    XFactor xh{};                   // static X half factor (no X coupling in the velocity gauge)
+    DBuf entries, oargs;
+    int64_t tiles_row = 0;
+    BatchFluxGeo geo{};
+  };
+  std::vector<Group> groups;
+  DBuf targs, ctr;
+  int64_t launched = 0;             // steps issued so far (host mirror of the device counter)
+  int64_t tmax = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
+  hipStream_t st() const { return ctx->stream; }
+
+  Batch(qsf_ctx* c, const qsf_grid_desc* descs, const qsf_field_desc* fields, int ns, const qsf_prop_desc& p)
+      : ctx(c), pd(p) {
+    QSF_REQUIRE(ns >= 1, QSF_ERR_ARG, "empty batch");
+    QSF_REQUIRE(p.mode == QSF_MODE_RE && p.order == 1 && p.gauge != QSF_GAUGE_LENGTH, QSF_ERR_UNSUPPORTED,
+                "batched solves run real-time Strang steps in the velocity gauge");
+    QSF_REQUIRE(p.nborders >= 0 && p.nborders <= 16, QSF_ERR_ARG, "0..16 borders");
+    prec = descs[0].precision;
+    solves.resize(ns);
+    for (int s = 0; s < ns; ++s) {
+      const qsf_grid_desc& d = descs[s];
+      QSF_REQUIRE(d.dim == 2 && !d.multigrid && d.precision == prec, QSF_ERR_UNSUPPORTED,
+                  "a batch holds single-region 2D grids of one precision");
+      Solve& so = solves[s];
+      so.g.reset(new Grid(c, d));
+      // host schedule: SplitPropagator RE loop of Strang steps (propagator.h:153, 311-346, 443-451)
+      phys::Coupling coup(QSF_GAUGE_VELOCITY, &fields[s], 1);
+      so.max_steps = (int64_t)(coup.max_pulse_duration() / p.dt + 1);
+      so.total = so.max_steps + 2;
+      so.coup.assign(so.total * 3, 0.0);
+      so.times.assign(so.total + 1, 0.0);
+      double timer = 0.0;
+      for (int64_t k = 0; k < so.total; ++k) {
+        coup.precalc(timer);            // the P substep's precalc (X substeps do not couple)
+        coup.values(&so.coup[3 * k]);
+        timer += p.dt * (1.0 * 1.0);
+        so.times[k + 1] = timer;
+      }
+      tmax = std::max(tmax, so.total);
+      so.coupling.ensure(so.coup.size() * sizeof(double));
+      QSF_HIP(hipMemcpy(so.coupling.p, so.coup.data(), so.coup.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // groups of identical grid descriptions
+    for (int s = 0; s < ns; ++s) {
+      int gi = -1;
+      for (size_t k = 0; k < groups.size(); ++k)
+        if (std::memcmp(&solves[groups[k].members[0]].g->d, &descs[s], sizeof(qsf_grid_desc)) == 0) gi = (int)k;
+      if (gi < 0) { groups.emplace_back(); gi = (int)groups.size() - 1; }
+      groups[gi].members.push_back(s);
+      solves[s].group = gi;
+    }
+    const double zero[3] = {0, 0, 0};
+    std::vector<BatchTableArgs> ta(ns);
+    for (auto& G : groups) {
+      G.lead = solves[G.members[0]].g.get();
+      Grid& L = *G.lead;
+      G.xh = L.x_factor(QSF_MODE_RE, 0.5 * p.dt, zero, false);   // built once, never recycled
+      L.build_tables();
+      if (p.nborders > 0) L.build_gather(p.borders, p.nborders);
+      PassParams rp = L.params(1, L.psi.p, L.psi.p);
+      G.tiles_row = axis_pass_tiles(prec, rp, ctx->variant_of(prec, 1, L.n[1], L.m));
+      std::vector<BatchEntry> be(G.members.size());
+      std::vector<BatchObsArgs> oa(G.members.size());
+      for (size_t k = 0; k < G.members.size(); ++k) {
+        Solve& so = solves[G.members[k]];
+        Grid& g = *so.g;
+        so.ptab.ensure(2 * std::max(g.n[0], g.n[1]) * g.cb);
+        so.partials.ensure(std::max<int64_t>(G.tiles_row, 1) * 3 * sizeof(double));
+        so.rec.ensure((so.total + 1) * (1 + 16) * sizeof(double));
+        so.fluxint.ensure(16 * sizeof(double));
+        QSF_HIP(hipMemset(so.fluxint.p, 0, 16 * sizeof(double)));
+        if (p.nborders > 0) so.gathered.ensure(std::max<int64_t>(L.gather->nvals, 1) * g.cb);
+        char* pt = static_cast<char*>(so.ptab.p);
+        const int64_t half = std::max(g.n[0], g.n[1]) * g.cb;
+        be[k] = BatchEntry{g.psi.p, {pt, pt + half, nullptr}, so.partials.as<double>(),
+                           p.nborders > 0 ? so.gathered.p : nullptr, so.total};
+        BatchTableArgs& t = ta[G.members[k]];
+        t = BatchTableArgs{};
+        for (int a = 0; a < 2; ++a) {
+          const int sa = g.pperm[a];
+          t.out[a] = a == 0 ? (void*)pt : (void*)(pt + half);
+          t.n[a] = g.n[a];
+          t.pn[a] = g.n[sa];
+          t.pn2[a] = g.n2[sa];
+          t.dp[a] = g.dp[sa];
+          t.storage_axis[a] = sa;
+        }
+        t.delta = p.dt;
+        t.coupling = so.coupling.as<double>();
+        t.total_steps = so.total;
+        oa[k] = BatchObsArgs{so.partials.as<double>(), G.tiles_row, so.gathered.p, so.coupling.as<double>(),
+                             so.total, so.rec.as<double>(), so.fluxint.as<double>(), g.dV};
+      }
+      G.entries.ensure(be.size() * sizeof(BatchEntry));
+      QSF_HIP(hipMemcpy(G.entries.p, be.data(), be.size() * sizeof(BatchEntry), hipMemcpyHostToDevice));
+      G.oargs.ensure(oa.size() * sizeof(BatchObsArgs));
+      QSF_HIP(hipMemcpy(G.oargs.p, oa.data(), oa.size() * sizeof(BatchObsArgs), hipMemcpyHostToDevice));
+      G.geo = BatchFluxGeo{};
+      G.geo.D = 2;
+      G.geo.precision = prec;
+      G.geo.dt = p.dt;
+      G.geo.nb = p.nborders;
+      G.geo.scale = L.dV / L.dx[0];
+      for (int a = 0; a < 3; ++a) G.geo.inv_2dx[a] = a < 2 ? (1.0 / L.dx[a]) / 2.0 : 0.0;
+      if (p.nborders > 0) {
+        G.geo.pts = L.gather->pts.as<int32_t>();
+        G.geo.bvec = L.gather->bvec.as<double>();
+        G.geo.npts = L.gather->npts;
+      }
+    }
+    targs.ensure(ta.size() * sizeof(BatchTableArgs));
+    QSF_HIP(hipMemcpy(targs.p, ta.data(), ta.size() * sizeof(BatchTableArgs), hipMemcpyHostToDevice));
+    ctr.ensure(sizeof(int64_t));
+    const int64_t z = 0;
+    QSF_HIP(hipMemcpy(ctr.p, &z, sizeof(int64_t), hipMemcpyHostToDevice));
+  }
+
+  ~Batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+
+  int64_t nmax() const {
+    int64_t v = 0;
+    for (auto& s : solves) v = std::max({v, s.g->n[0], s.g->n[1]});
+    return v;
+  }
+
+  // observables of the starting states (computeEach<AT_START>)
+  void start() {
+    for (auto& so : solves) so.start_row = observe_now(so, 0);
+  }
+
+  std::vector<double> observe_now(Solve& so, int64_t k) {
+    Grid& g = *so.g;
+    std::vector<double> row(2 + 1 + pd.nborders, std::nan(""));
+    row[0] = (double)k;
+    row[1] = so.times[k];
+    double r3[3];
+    g.observe(false, false, r3);
+    row[2] = r3[0] * g.dV;
+    if (pd.nborders > 0) {
+      const double* C = k > 0 ? &so.coup[3 * (k - 1)] : nullptr;
+      g.flux(C, pd.borders, pd.nborders, &row[3]);
+    }
+    return row;
+  }
+
+  // one step of every active solve; the first step has no pending inverse / X / observation
+  void enqueue_step(bool first) {
+    launch_batch_ptables(prec, targs.as<BatchTableArgs>(), (int)solves.size(), nmax(), ctr.as<int64_t>(), st());
+    for (auto& G : groups) {
+      Grid& L = *G.lead;
+      PassParams rp = L.params(1, L.psi.p, L.psi.p);
+      rp.batch = G.entries.as<BatchEntry>();
+      rp.step_ctr = ctr.as<int64_t>();
+      rp.nbatch = (int32_t)G.members.size();
+      rp.tw_mode = ctx->tw_mode[prec];
+      rp.flags = F_XB | F_FWD;
+      rp.xb = G.xh;
+      if (!first) {
+        rp.flags |= F_INV1 | F_XA | F_OBS;
+        rp.xa = G.xh;
+        L.obs_params(rp.obs, false);
+        if (pd.nborders > 0) L.set_gather(rp.obs);
+      }
+      int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, 1, L.n[1], L.m));
+      QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
+      if (!first) launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), st());
+      PassParams cp = L.params(0, L.psi.p, L.psi.p);
+      cp.batch = G.entries.as<BatchEntry>();
+      cp.step_ctr = ctr.as<int64_t>();
+      cp.nbatch = (int32_t)G.members.size();
+      cp.tw_mode = ctx->tw_mode[prec];
+      cp.flags = F_FWD | F_P | F_INV2;
+      k = launch_axis_pass(prec, cp, st(), ctx->variant_of(prec, 0, L.n[0], L.m));
+      QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
+    }
+    launch_step_increment(ctr.as<int64_t>(), st());
+    QSF_HIP(hipGetLastError());
+  }
+
+  void run(int64_t k, bool use_graph) {
+    while (k > 0 && launched < tmax) {
+      if (launched == 0) {
+        enqueue_step(true);
+      } else if (use_graph) {
+        if (!gexec) {
+          QSF_HIP(hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
+          enqueue_step(false);
+          QSF_HIP(hipStreamEndCapture(st(), &graph));
+          QSF_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        QSF_HIP(hipGraphLaunch(gexec, st()));
+      } else {
+        enqueue_step(false);
+      }
+      ++launched;
+      --k;
+    }
+  }
+
+  bool done() const { return launched >= tmax; }
+
+  // After the last step: apply each solve's pending inverse + X half and observe the end state
+  void finish() {
+    for (auto& so : solves) {
+      Grid& g = *so.g;
+      PassParams p = g.params(1, g.psi.p, g.psi.p);
+      p.flags = F_INV1 | F_XA;
+      p.xa = groups[so.group].xh;
+      g.run(p);
+      so.end_row = observe_now(so, so.total);
+    }
+  }
+
+  std::vector<double> records(int s) {
+    Solve& so = solves[s];
+    const int nb = pd.nborders, w = 3 + nb;
+    std::vector<double> dev((so.total + 1) * (1 + 16));
+    QSF_HIP(hipStreamSynchronize(st()));
+    QSF_HIP(hipMemcpy(dev.data(), so.rec.p, dev.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> out;
+    const int64_t upto = std::min<int64_t>(launched - 1, so.total - 1);
+    if (!so.start_row.empty()) out.insert(out.end(), so.start_row.begin(), so.start_row.end());
+    for (int64_t k = 1; k <= upto; ++k) {
+      std::vector<double> row(w);
+      row[0] = (double)k;
+      row[1] = so.times[k];
+      for (int j = 0; j < 1 + nb; ++j) row[2 + j] = dev[k * (1 + nb) + j];
+      out.insert(out.end(), row.begin(), row.end());
+    }
+    if (!so.end_row.empty()) out.insert(out.end(), so.end_row.begin(), so.end_row.end());
+    return out;
+  }
+};
+
 }  // namespace qsf
 
 using namespace qsf;
@@ -1363,6 +1639,9 @@ using namespace qsf;
 struct qsf_set { Set s; qsf_set(qsf_ctx* c, const qsf_grid_desc& d) : s(c, d) {} };
 struct qsf_grid { Grid* g; };
 struct qsf_prop { Prop p; qsf_prop(Set* s, const qsf_prop_desc& d, const qsf_field_desc* f, int nf) : p(s, d, f, nf) {} };
+
+struct qsf_batch { Batch b; qsf_batch(qsf_ctx* c, const qsf_grid_desc* d, const qsf_field_desc* f, int n,
+                                         const qsf_prop_desc& p) : b(c, d, f, n, p) {} };
 
 namespace {
 thread_local std::string g_noctx_err;
@@ -1700,6 +1979,77 @@ int qsf_prop_flux_integral(qsf_prop* p, double* out) {
     QSF_HIP(hipStreamSynchronize(p->p.st()));
   });
 }
+int qsf_batch_create(qsf_ctx* c, const qsf_grid_desc* grids, const qsf_field_desc* fields, int n,
+                     const qsf_prop_desc* p, qsf_batch** out) {
+  if (!c || !grids || !fields || !p || !out || n < 1) return QSF_ERR_ARG;
+  *out = nullptr;
+  return guard(c, [&] { *out = new qsf_batch(c, grids, fields, n, *p); });
+}
+int qsf_batch_destroy(qsf_batch* b) {
+  if (!b) return QSF_ERR_ARG;
+  (void)hipStreamSynchronize(b->b.st());
+  delete b;
+  return QSF_OK;
+}
+int qsf_batch_upload(qsf_batch* b, int s, const double* psi) {
+  if (!b || !psi || s < 0 || s >= (int)b->b.solves.size()) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] {
+    QSF_REQUIRE(b->b.launched == 0, QSF_ERR_STATE, "upload before the first step");
+    Grid& g = *b->b.solves[s].g;
+    g.upload(psi, g.m, QSF_REP_X);
+  });
+}
+int qsf_batch_download(qsf_batch* b, int s, double* psi) {
+  if (!b || !psi || s < 0 || s >= (int)b->b.solves.size()) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] {
+    QSF_REQUIRE(b->b.launched == 0 || !b->b.solves[s].end_row.empty(), QSF_ERR_STATE,
+                "download after qsf_batch_finish (the state is mid-pipeline)");
+    Grid& g = *b->b.solves[s].g;
+    g.download(psi, g.m);
+  });
+}
+int qsf_batch_start(qsf_batch* b) {
+  if (!b) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] { b->b.start(); });
+}
+int qsf_batch_run(qsf_batch* b, int64_t k, int use_graph, int* done) {
+  if (!b) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] {
+    b->b.run(k, use_graph != 0);
+    if (done) *done = b->b.done();
+  });
+}
+int qsf_batch_finish(qsf_batch* b) {
+  if (!b) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] {
+    QSF_REQUIRE(b->b.done(), QSF_ERR_STATE, "every solve must have taken its steps");
+    b->b.finish();
+  });
+}
+int qsf_batch_info(qsf_batch* b, int s, int64_t* max_steps, int64_t* grid_size) {
+  if (!b || s < 0 || s >= (int)b->b.solves.size()) return QSF_ERR_ARG;
+  if (max_steps) *max_steps = b->b.solves[s].max_steps;
+  if (grid_size) *grid_size = b->b.solves[s].g->m;
+  return QSF_OK;
+}
+int qsf_batch_records(qsf_batch* b, int s, double* out, int max_rows, int* rows) {
+  if (!b || s < 0 || s >= (int)b->b.solves.size()) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] {
+    auto r = b->b.records(s);
+    const int w = 3 + b->b.pd.nborders;
+    const int n = (int)(r.size() / w);
+    if (rows) *rows = n;
+    if (out) std::memcpy(out, r.data(), std::min(n, max_rows) * w * sizeof(double));
+  });
+}
+int qsf_batch_flux_integral(qsf_batch* b, int s, double* out) {
+  if (!b || !out || s < 0 || s >= (int)b->b.solves.size()) return QSF_ERR_ARG;
+  return guard(b->b.ctx, [&] {
+    QSF_HIP(hipStreamSynchronize(b->b.st()));
+    QSF_HIP(hipMemcpy(out, b->b.solves[s].fluxint.p, b->b.pd.nborders * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
 int qsf_prop_set_exec(qsf_prop* p, int mode) {
   if (!p || (mode != 0 && mode != 1)) return QSF_ERR_ARG;
   return guard(p->p.set->ctx, [&] {

@@ -65,6 +65,17 @@ SIGNATURES = {
     "qsf_tune_entry": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I64), C.POINTER(_I64),
                             C.POINTER(_I), _dp]),
     "qsf_tune_set": (_I, [_P, _I, _I, _I64, _I64, _I]),
+    "qsf_batch_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(T.FieldDesc), _I, C.POINTER(T.PropDesc),
+                              C.POINTER(_P)]),
+    "qsf_batch_destroy": (_I, [_P]),
+    "qsf_batch_upload": (_I, [_P, _I, _dp]),
+    "qsf_batch_download": (_I, [_P, _I, _dp]),
+    "qsf_batch_start": (_I, [_P]),
+    "qsf_batch_run": (_I, [_P, _I64, _I, C.POINTER(_I)]),
+    "qsf_batch_finish": (_I, [_P]),
+    "qsf_batch_info": (_I, [_P, _I, C.POINTER(_I64), C.POINTER(_I64)]),
+    "qsf_batch_records": (_I, [_P, _I, _dp, _I, C.POINTER(_I)]),
+    "qsf_batch_flux_integral": (_I, [_P, _I, _dp]),
 }
 
 
@@ -411,3 +422,76 @@ class Propagator:
         if limit is None or k < limit:
             self.finish()
         return self.records()
+
+
+class Batch:
+    """Many single-region 2D solves advanced together (batched launches, hipGraph replay)."""
+
+    def __init__(self, ctx, descs, fields, pdesc):
+        self.ctx, self._pd = ctx, pdesc
+        n = len(descs)
+        self._descs = (T.GridDesc * n)(*descs)
+        self._fields = (T.FieldDesc * n)(*fields)
+        h = _P()
+        ctx.check(load().qsf_batch_create(ctx.h, self._descs, self._fields, n, C.byref(pdesc), C.byref(h)),
+                  "batch_create")
+        self.h, self.n = h, n
+        ctx.adopt(self)
+        self.sizes = []
+        for s in range(n):
+            ms, m = C.c_int64(), C.c_int64()
+            ctx.check(load().qsf_batch_info(h, s, C.byref(ms), C.byref(m)), "batch_info")
+            self.sizes.append((ms.value, m.value))
+
+    def close(self):
+        if getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None):
+            load().qsf_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_psi(self, s, psi):
+        a = np.ascontiguousarray(np.asarray(psi, dtype=np.complex128)).ravel()
+        assert a.size == self.sizes[s][1]
+        self.ctx.check(load().qsf_batch_upload(self.h, s, _dptr(a)), "batch_upload")
+
+    def psi(self, s):
+        a = np.empty(self.sizes[s][1], np.complex128)
+        self.ctx.check(load().qsf_batch_download(self.h, s, _dptr(a)), "batch_download")
+        return a
+
+    def start(self):
+        self.ctx.check(load().qsf_batch_start(self.h), "batch_start")
+
+    def run(self, steps, use_graph=True):
+        done = C.c_int()
+        self.ctx.check(load().qsf_batch_run(self.h, steps, 1 if use_graph else 0, C.byref(done)), "batch_run")
+        return bool(done.value)
+
+    def finish(self):
+        self.ctx.check(load().qsf_batch_finish(self.h), "batch_finish")
+
+    def records(self, s):
+        rows = C.c_int()
+        self.ctx.check(load().qsf_batch_records(self.h, s, None, 0, C.byref(rows)), "batch_records")
+        w = 3 + self._pd.nborders
+        out = np.zeros(rows.value * w)
+        if rows.value:
+            self.ctx.check(load().qsf_batch_records(self.h, s, _dptr(out), rows.value, C.byref(rows)),
+                           "batch_records")
+        return out.reshape(rows.value, w)
+
+    def flux_integral(self, s):
+        out = np.zeros(max(1, self._pd.nborders))
+        self.ctx.check(load().qsf_batch_flux_integral(self.h, s, _dptr(out)), "batch_flux_integral")
+        return out[: self._pd.nborders]
+
+    def run_all(self, use_graph=True, chunk=1 << 30):
+        self.start()
+        while not self.run(chunk, use_graph):
+            pass
+        self.finish()

@@ -323,3 +323,35 @@ def test_grid_teardown_with_unbuilt_tables(ctx):
         p.close()
         g.close()
     ctx.synchronize()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_single_solves(ctx, use_graph):
+    # uneven sizes and fields in one batch; every solve equals its own single-solve propagation
+    # (same kernels: up to summation order of the observables) and the FP64 oracle
+    from qsf_amd import hip
+    base = dict(potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                absorber=T.ABS_CAP, nCAP=8)
+    sizes = [64, 96, 64, 128, 48]
+    descs = [T.grid_desc(2, n, 0.3, **base) for n in sizes]
+    fields = [T.field_desc(T.AXIS_XY, 0.05 + 0.02 * i, 2.0, 1 + (i % 2), phase_in_pi=0.3 * i)
+              for i in range(len(sizes))]
+    borders = [(b, 0) for b in T.ZOA_FLUX_2D] + [(T.BORDER_BOX, 18)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_VELOCITY, outputs=T.OUT_NORM | T.OUT_FLUX,
+                        borders=borders)
+    psis = [crand(n * n) * 0.01 for n in sizes]
+    b = hip.Batch(ctx, descs, fields, pdesc)
+    for s, p in enumerate(psis):
+        b.set_psi(s, p)
+    b.run_all(use_graph=use_graph)
+    for s, n in enumerate(sizes):
+        print(f"[case] batch solve {s} n={n}", flush=True)
+        ref = O.GridSet(descs[s])
+        ref[0].set_psi(psis[s])
+        ro = O.Propagator(ref, pdesc, [fields[s]]).run()
+        rb = b.records(s)
+        assert rb.shape[0] == ro.shape[0]
+        assert rel(b.psi(s), ref[0].psi()) < FP64_TOL
+        assert np.allclose(rb[:, 2], ro[:, T.REC_NORM], rtol=1e-11)
+        assert np.allclose(rb[:, 3:], ro[:, T.REC_FLUX0:], rtol=1e-8, atol=1e-12 * np.abs(ro[:, T.REC_FLUX0:]).max())
+    b.close()
