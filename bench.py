@@ -63,13 +63,19 @@ def initial_state(m, seed=0):
     return (rng.normal(size=m) + 1j * rng.normal(size=m)) * 1e-3
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_device(desc, pdesc, fields, steps, warmup, profile=True):
     from qsf_amd import hip
+    progress(f"device run: precision {desc.precision}, {steps} steps after {warmup} warm-up")
     ctx = hip.Context(0)
     gset = hip.GridSet(ctx, desc)
     gset[0].set_psi(initial_state(gset[0].m))
     for g in gset.grids:   # pick the fastest compiled kernel variant of every pass (untimed)
         g.autotune(5)
+    progress("autotuned; warming up")
     prop = hip.Propagator(gset, pdesc, fields)
     prop.start()
     prop.run_steps(warmup)
@@ -97,21 +103,63 @@ def run_device(desc, pdesc, fields, steps, warmup, profile=True):
                 last=rec[-1].tolist() if len(rec) else None)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summaries
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy's readings), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_pmc_traffic.txt"))):
+        take = False
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                take = True
+                continue
+            parts = line.split()
+            if take and len(parts) == 2 and parts[0] == kernel:
+                return int(parts[1]), os.path.basename(path)
+    return None, None
+
+
 def roofline_of(kernels):
     if not kernels:
         return None
     name, (k, ms, by) = max(kernels.items(), key=lambda kv: kv[1][1])
     avg_ms = ms / k
     achieved = by / (avg_ms * 1e-3) / 1e9
+    traffic, source = measured_traffic(name)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source,
+            "kernel": name,
             "avg_launch_us": round(avg_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(by),
             "launches": int(k)}
+
+
+def copy_calibration(steps):
+    """A plain device-to-device copy of a known size (256 MiB). Its FETCH_SIZE / WRITE_SIZE
+    readings under rocprofv3 --pmc give the scale that turns those counters into HBM bytes."""
+    import torch
+    nbytes = 256 << 20
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    a.normal_()
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"metric": "copy GB/s", "value": round(2 * nbytes / (ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+            "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+            "ms_per_launch": round(ms, 4), "launches": steps + 3}
 
 
 def cpu_baseline(desc, pdesc, fields, budget_s=20.0):
     """The FP64 oracle (one host core) on whole steps of the same workload, bounded in time."""
     from oracle import oracle as O
+    progress(f"cpu baseline (oracle, one core, about {budget_s / 2:.0f} s)")
     gs = O.GridSet(desc)
     gs[0].set_psi(initial_state(gs[0].m))
     prop = O.Propagator(gs, pdesc, fields)
@@ -134,7 +182,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["c2", "c3"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "copy"], default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
@@ -143,6 +191,9 @@ def main():
               "is not supported in this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "copy":
+        print(json.dumps(copy_calibration(args.steps)))
+        return
     if args.workload == "c2":
         desc, pdesc, fields = c2_desc(T.FP64), c2_prop(), C2_FIELDS
         workload = ("2D two-electron reduced model 2048x2048 dx=0.2, Eberly, velocity gauge, "
