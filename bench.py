@@ -58,6 +58,89 @@ def c3_fields():
     return [T.field_desc(T.AXIS_XYZ, E0, 0.057, 2)]
 
 
+C4_SIDES = (512, 768, 1024, 1536, 2048)
+
+
+def c4_solves(precision, count=64, seed=4):
+    """64 independent 2D two-electron solves: sides U{512..2048}, intensity log-U[5e13, 1e15]
+    W/cm^2 (E0 = sqrt(I / 3.50944e16)), CEP U[0, 2pi), 2-6 cycles at 800 nm; the 2048^2
+    workload's physics without multigrid (Strang, dt=0.02, velocity gauge, cos^(1/8))."""
+    rng = np.random.default_rng(seed)
+    sides = rng.choice(C4_SIDES, size=count)
+    intensity = 10.0 ** rng.uniform(np.log10(5e13), np.log10(1e15), size=count)
+    e0 = np.sqrt(intensity / 3.50944e16)
+    cep = rng.uniform(0.0, 2.0, size=count)           # phase in units of pi
+    cycles = rng.integers(2, 7, size=count)
+    descs = [c2_desc(precision, multigrid=0, n=int(n)) for n in sides]
+    fields = [T.field_desc(T.AXIS_XY, float(e), 0.057, int(c), phase_in_pi=float(p))
+              for e, c, p in zip(e0, cycles, cep)]
+    return descs, fields
+
+
+def c4_prop(order=1):
+    return T.prop_desc(T.MODE_RE, order, 0.02, gauge=T.GAUGE_VELOCITY,
+                       outputs=T.OUT_NORM | T.OUT_FLUX, borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+
+
+def ground_like_state(n, dx=0.2):
+    """A bound two-electron-like packet (product of soft-core 1s-shaped orbitals)."""
+    x = -0.5 * dx * (n - 1) + dx * np.arange(n)
+    f = np.exp(-np.sqrt(x * x + 0.5))
+    psi = np.outer(f, f).ravel().astype(np.complex128)
+    return psi / np.sqrt(np.sum(np.abs(psi) ** 2) * dx * dx)
+
+
+def run_c4(steps, warmup):
+    """The batched FP32-mixed step of 64 uneven solves (hipGraph replay) beside the plain FP64
+    step of the same solves run one after another."""
+    from qsf_amd import hip
+    ctx = hip.Context(0)
+    descs32, fields = c4_solves(T.FP32_MIXED)
+    pd = c4_prop()
+    # autotune every size once for both precisions (single grids share the tuning table)
+    for n in C4_SIDES:
+        for prec in (T.FP32_MIXED, T.FP64):
+            gs = hip.GridSet(ctx, c2_desc(prec, multigrid=0, n=n))
+            gs[0].set_psi(ground_like_state(n))
+            gs[0].autotune(5)
+            gs.close()
+    progress("autotuned; building the batch")
+    b = hip.Batch(ctx, descs32, fields, pd)
+    for s, d in enumerate(descs32):
+        b.set_psi(s, ground_like_state(d.n[0]))
+    b.start()
+    b.run(warmup, use_graph=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps, use_graph=True)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    pts = sum(int(d.n[0]) * int(d.n[1]) for d in descs32)
+    batched = pts * steps / wall
+    b.close()
+    # the plain FP64 step: one propagator per solve, per-size step time
+    per_size = {}
+    for n in C4_SIDES:
+        gs = hip.GridSet(ctx, c2_desc(T.FP64, multigrid=0, n=n))
+        gs[0].set_psi(ground_like_state(n))
+        p = hip.Propagator(gs, pd, [fields[0]])
+        p.start()
+        p.run_steps(warmup)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        p.run_steps(steps)
+        ctx.synchronize()
+        per_size[n] = (time.perf_counter() - t1) / steps
+        p.close()
+        gs.close()
+    plain_s = sum(per_size[int(d.n[0])] for d in descs32)
+    plain = pts / plain_s
+    ctx.close()
+    return dict(value=batched, ms_per_step=wall / steps * 1e3, pts=pts, plain=plain,
+                plain_ms_per_step=plain_s * 1e3, per_size_ms={k: round(v * 1e3, 4) for k, v in per_size.items()},
+                sides={int(n): int(sum(1 for d in descs32 if d.n[0] == n)) for n in C4_SIDES})
+
+
 def initial_state(m, seed=0):
     rng = np.random.default_rng(seed)
     return (rng.normal(size=m) + 1j * rng.normal(size=m)) * 1e-3
@@ -182,7 +265,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["c2", "c3", "copy"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "copy"], default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
@@ -193,6 +276,22 @@ def main():
 
     if args.workload == "copy":
         print(json.dumps(copy_calibration(args.steps)))
+        return
+    if args.workload == "c4":
+        r = run_c4(args.steps, args.warmup)
+        out = {"metric": "grid-point-steps/s", "value": round(r["value"], 1), "unit": "grid-point-steps/s",
+               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "c64 (f32, f64 phases and reductions)", "data": "synthetic",
+               "config": {"workload": "64 independent 2D two-electron solves, sides U{512..2048}, "
+                                      "I log-U[5e13,1e15] W/cm^2, CEP U[0,2pi), 2-6 cycles, Strang dt=0.02, "
+                                      "velocity gauge, cos^(1/8), norm + ZOA_FLUX_2D every step; batched "
+                                      "FP32-mixed step replayed as a hipGraph",
+                          "grid_points_per_step": r["pts"], "solves_per_side": r["sides"]},
+               "plain_fp64": {"value": round(r["plain"], 1), "ms_per_step": round(r["plain_ms_per_step"], 4),
+                              "per_size_ms_per_step": r["per_size_ms"]},
+               "speedup_vs_plain_fp64": round(r["value"] / r["plain"], 3)}
+        print(json.dumps(out))
         return
     if args.workload == "c2":
         desc, pdesc, fields = c2_desc(T.FP64), c2_prop(), C2_FIELDS

@@ -63,8 +63,10 @@ void launch_axis_tables(int precision, const AxisTableArgs* a, int count, hipStr
       nmax = a[off + i].n > nmax ? a[off + i].n : nmax;
     }
     dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)k);
-    if (precision == QSF_FP64) hipLaunchKernelGGL(axis_tables_kernel<double>, grid, dim3(256), 0, s, b);
-    else hipLaunchKernelGGL(axis_tables_kernel<float>, grid, dim3(256), 0, s, b);
+    // phase tables are FP64 in both precisions: the FP32 step applies them with one FP64
+    // product and a single rounding (a table rounded to FP32 once would bias every step)
+    (void)precision;
+    hipLaunchKernelGGL(axis_tables_kernel<double>, grid, dim3(256), 0, s, b);
   }
 }
 
@@ -94,8 +96,8 @@ __global__ void pair_table_kernel(PairTableArgs a) {
 void launch_pair_table(int precision, const PairTableArgs& a, hipStream_t s) {
   const int64_t total = a.kind == 1 ? a.n : a.n * a.n;
   dim3 grid((unsigned)((total + 255) / 256));
-  if (precision == QSF_FP64) hipLaunchKernelGGL(pair_table_kernel<double>, grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(pair_table_kernel<float>, grid, dim3(256), 0, s, a);
+  (void)precision;  // FP64 in both precisions (see launch_axis_tables)
+  hipLaunchKernelGGL(pair_table_kernel<double>, grid, dim3(256), 0, s, a);
 }
 
 template <typename Real>
@@ -104,7 +106,10 @@ __global__ void roots_kernel(typename cvec<Real>::T* out, int64_t n) {
   if (j >= n) return;
   double s, c;
   sincospi(-2.0 * (double)j / (double)n, &s, &c);
-  out[j] = typename cvec<Real>::T{(Real)c, (Real)s};
+  const Real hc = (Real)c, hs = (Real)s;
+  out[j] = typename cvec<Real>::T{hc, hs};
+  // FP32: the rounding remainders follow the n roots (hi + lo carries the FP64 root)
+  if constexpr (sizeof(Real) == 4) out[n + j] = typename cvec<Real>::T{(Real)(c - (double)hc), (Real)(s - (double)hs)};
 }
 
 void launch_roots(int precision, void* out, int64_t n, hipStream_t s) {
@@ -353,8 +358,8 @@ __global__ void batch_ptables_kernel(const BatchTableArgs* args, const int64_t* 
 void launch_batch_ptables(int precision, const BatchTableArgs* dev_args, int nsolves, int64_t nmax,
                           const int64_t* step_ctr, hipStream_t s) {
   dim3 grid((unsigned)((nmax + 255) / 256), (unsigned)nsolves, 2);
-  if (precision == QSF_FP64) hipLaunchKernelGGL(batch_ptables_kernel<double>, grid, dim3(256), 0, s, dev_args, step_ctr);
-  else hipLaunchKernelGGL(batch_ptables_kernel<float>, grid, dim3(256), 0, s, dev_args, step_ctr);
+  (void)precision;  // FP64 in both precisions (see launch_axis_tables)
+  hipLaunchKernelGGL(batch_ptables_kernel<double>, grid, dim3(256), 0, s, dev_args, step_ctr);
 }
 
 // One block per solve: norm from the pass partials and the flux families from the gathered

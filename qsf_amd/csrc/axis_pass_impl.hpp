@@ -75,23 +75,14 @@ __device__ __forceinline__ T ld(const void* tab, int64_t i) {
 
 // Per-thread evaluator of an X factor for the elements of one transform along `axis`
 // (scalar members only: runtime-indexed arrays would live in scratch memory).
-// exp(-i v) (RE) or exp(-v) (IM) for an FP64 argument; in FP32 the argument is reduced
-// modulo 2π in FP64 before the single-precision sincos.
-template <typename T>
-__device__ __forceinline__ T expop(double v, int mode) {
-  using R = decltype(T{}.x);
-  if (mode == 1) return T{(R)exp(-v), R(0)};   // QSF_MODE_IM
-  if constexpr (sizeof(R) == 8) {
-    double sn, cs;
-    sincos(-v, &sn, &cs);
-    return T{cs, sn};
-  } else {
-    const double twopi = 6.283185307179586476925286766559;
-    const double r = v - twopi * rint(v * (1.0 / twopi));
-    float sn, cs;
-    __sincosf(-(float)r, &sn, &cs);
-    return T{cs, sn};
-  }
+// Factors are complex FP64 in both precisions (tables, products, closed forms); the state
+// meets them once per element through capply (one rounding to the state type).
+// exp(-i v) (RE) or exp(-v) (IM) for an FP64 argument.
+__device__ __forceinline__ double2 expop(double v, int mode) {
+  if (mode == 1) return double2{exp(-v), 0.0};   // QSF_MODE_IM
+  double sn, cs;
+  sincos(-v, &sn, &cs);
+  return double2{cs, sn};
 }
 
 __device__ __forceinline__ double ee_closed(double x, double y, int model, double EE, double Esoft) {
@@ -99,7 +90,8 @@ __device__ __forceinline__ double ee_closed(double x, double y, int model, doubl
   return EE / sqrt((x - y) * (x - y) + Esoft);                          // Eberly
 }
 
-template <typename T, int N> struct XEval {
+template <int N> struct XEval {
+  using T = double2;
   T cst;
   const T* ax;
   const T* pr0;
@@ -110,9 +102,8 @@ template <typename T, int N> struct XEval {
   double delta, EE, Esoft, xmin, dx;
   int mode, model;
   __device__ void init(const XFactor& f, int axis, int i0, int i1, int i2) {
-    using R = decltype(cst.x);
     auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
-    cst = T{R(1), R(0)};
+    cst = T{1.0, 0.0};
     ax = reinterpret_cast<const T*>(axis == 0 ? f.axis[0] : (axis == 1 ? f.axis[1] : f.axis[2]));
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -145,7 +136,7 @@ template <typename T, int N> struct XEval {
           ++npr;
         }
       }
-      if (any) cst = cmul(cst, expop<T>(delta * sc, mode));
+      if (any) cst = cmul(cst, expop(delta * sc, mode));
     } else if (kind != PAIR_NONE) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -166,14 +157,13 @@ template <typename T, int N> struct XEval {
   }
   // product of the pair terms that involve the transform axis (1 if none)
   __device__ __forceinline__ T pair_at(int n) const {
-    using R = decltype(cst.x);
-    T r{R(1), R(0)};
+    T r{1.0, 0.0};
     if (kind == PAIR_ONFLY) {
       if (npr == 0) return r;
       const double xn = xmin + n * dx;
       double sv = ee_closed(xn, xmin + o0 * dx, model, EE, Esoft);
       if (npr > 1) sv += ee_closed(xn, xmin + o1 * dx, model, EE, Esoft);
-      return expop<T>(delta * sv, mode);
+      return expop(delta * sv, mode);
     }
     if (npr > 0) r = kind == PAIR_DIFF ? pr0[abs(n - o0)] : pr0[n];
     if (npr > 1) r = cmul(r, kind == PAIR_DIFF ? pr1[abs(n - o1)] : pr1[n]);
@@ -321,7 +311,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
   const uint32_t XF = p.flags & (F_XA | F_OBS | F_MASK | F_XB);
   if (XF) {
     const bool doA = p.flags & F_XA, doB = p.flags & F_XB, obs = p.flags & F_OBS, msk = p.flags & F_MASK;
-    XEval<T, N> xa, xb;
+    XEval<N> xa, xb;
     if (doA) xa.init(p.xa, p.axis, i0, i1, i2);
     if (doB) xb.init(p.xb, p.axis, i0, i1, i2);
     const bool shared = doA && doB && xa.npr > 0 && p.xa.pair_kind == p.xb.pair_kind &&
@@ -360,20 +350,22 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
       for (int q = 0; q < E; ++q) {
         const int ia = t + TT * q;
         if (shared) {
-          const T pp = xa.pair_at(ia);
-          v[q] = cmul(v[q], cmul(cmul(xa.base_at(ia), xb.base_at(ia)), cmul(pp, pp)));
+          const double2 pp = xa.pair_at(ia);
+          v[q] = capply(v[q], cmul(cmul(xa.base_at(ia), xb.base_at(ia)), cmul(pp, pp)));
+        } else if (doA && doB) {
+          v[q] = capply(v[q], cmul(xa.at(ia), xb.at(ia)));
         } else {
-          if (doA) v[q] = cmul(v[q], xa.at(ia));
-          if (doB) v[q] = cmul(v[q], xb.at(ia));
+          if (doA) v[q] = capply(v[q], xa.at(ia));
+          if (doB) v[q] = capply(v[q], xb.at(ia));
         }
       }
     } else
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int ia = t + TT * q;
-      T pp;
+      double2 pp;
       if (shared) pp = xa.pair_at(ia);
-      if (doA) v[q] = cmul(v[q], shared ? cmul(xa.base_at(ia), pp) : xa.at(ia));
+      if (doA) v[q] = capply(v[q], shared ? cmul(xa.base_at(ia), pp) : xa.at(ia));
       if (obs && valid) {
         const double a2 = (double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y;
         nrm += a2;
@@ -412,7 +404,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
         const double mk = p.mask_tab[off];
         v[q] = T{(Real)((double)v[q].x * mk), (Real)((double)v[q].y * mk)};
       }
-      if (doB) v[q] = cmul(v[q], shared ? cmul(xb.base_at(ia), pp) : xb.at(ia));
+      if (doB) v[q] = capply(v[q], shared ? cmul(xb.base_at(ia), pp) : xb.at(ia));
     }
     if (obs) {
       // block reduction (fixed order: lanes, then waves)
@@ -446,15 +438,16 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
   if (p.flags & F_FWD) run_fft(FWD{});
 
   if (p.flags & F_P) {
-    T cst{Real(1), Real(0)};
-    if (p.axis != 0 && pf0) cst = cmul(cst, ld<T>(pf0, i0));
-    if (p.axis != 1 && pf1) cst = cmul(cst, ld<T>(pf1, i1));
-    if (p.axis != 2 && pf2) cst = cmul(cst, ld<T>(pf2, i2));
-    const T* pa = reinterpret_cast<const T*>(p.axis == 0 ? pf0 : (p.axis == 1 ? pf1 : pf2));
+    // P phase: complex FP64 per-axis tables, one FP64 product per element
+    double2 cst{1.0, 0.0};
+    if (p.axis != 0 && pf0) cst = cmul(cst, ld<double2>(pf0, i0));
+    if (p.axis != 1 && pf1) cst = cmul(cst, ld<double2>(pf1, i1));
+    if (p.axis != 2 && pf2) cst = cmul(cst, ld<double2>(pf2, i2));
+    const double2* pa = reinterpret_cast<const double2*>(p.axis == 0 ? pf0 : (p.axis == 1 ? pf1 : pf2));
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      T f = pa ? cmul(cst, pa[t + TT * q]) : cst;
-      v[q] = cmul(v[q], f);
+      const double2 f = pa ? cmul(cst, pa[t + TT * q]) : cst;
+      v[q] = capply(v[q], f);
     }
   }
 

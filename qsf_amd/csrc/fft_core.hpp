@@ -30,6 +30,39 @@ template <int DIR, typename T> __device__ __forceinline__ T mul_di(T a) {
   return DIR < 0 ? T{a.y, -a.x} : T{-a.y, a.x};
 }
 
+// Products by constants and tables that are the same at every step. In FP32 a once-rounded
+// constant would bias every transform the same way (a norm drift linear in the step count), so
+// the FP32 forms carry the rounding remainder: c = c_hi + c_lo, v*c = v*c_hi + v*c_lo.
+template <typename T> __device__ __forceinline__ T cmul_k(T v, double cr, double ci) {
+  using R = decltype(v.x);
+  if constexpr (sizeof(R) == 8) {
+    return cmul(v, T{cr, ci});
+  } else {
+    const float hr = (float)cr, hi = (float)ci;
+    const float lr = (float)(cr - (double)hr), li = (float)(ci - (double)hi);
+    return cadd(cmul(v, T{lr, li}), cmul(v, T{hr, hi}));
+  }
+}
+template <typename T> __device__ __forceinline__ T cscale_k(T v, double c) {
+  using R = decltype(v.x);
+  if constexpr (sizeof(R) == 8) {
+    return T{v.x * c, v.y * c};
+  } else {
+    const float h = (float)c, l = (float)(c - (double)h);
+    return T{v.x * h + v.x * l, v.y * h + v.y * l};
+  }
+}
+// state times an FP64 phase factor: FP64 product, one rounding to the state type
+template <typename T> __device__ __forceinline__ T capply(T v, double2 z) {
+  using R = decltype(v.x);
+  if constexpr (sizeof(R) == 8) {
+    return cmul(v, z);
+  } else {
+    const double2 r = cmul(double2{(double)v.x, (double)v.y}, z);
+    return T{(float)r.x, (float)r.y};
+  }
+}
+
 // ---------------------------------------------------------------- small DFTs (in registers)
 template <int DIR, typename T> __device__ __forceinline__ void dft2(T& a, T& b) {
   T t = a;
@@ -40,10 +73,9 @@ template <int DIR, typename T> __device__ __forceinline__ void dft2(T& a, T& b) 
 template <int DIR, typename T> __device__ __forceinline__ void dft3(T& a, T& b, T& c) {
   using R = decltype(a.x);
   const R h = R(0.5);
-  const R s = R(0.86602540378443864676372317075294);  // sqrt(3)/2
   T t1 = cadd(b, c), t2 = csub(b, c);
   T m = T{a.x - h * t1.x, a.y - h * t1.y};
-  T r = mul_di<DIR>(cscale(t2, s));
+  T r = mul_di<DIR>(cscale_k(t2, 0.86602540378443864676372317075294));  // sqrt(3)/2
   a = cadd(a, t1);
   b = cadd(m, r);
   c = csub(m, r);
@@ -59,12 +91,11 @@ template <int DIR, typename T> __device__ __forceinline__ void dft4(T& a, T& b, 
 
 // w8^k for k = 1..3 with w8 = exp(DIR*2πi/8)
 template <int DIR, int K, typename T> __device__ __forceinline__ T tw8(T v) {
-  using R = decltype(v.x);
-  const R c = R(0.70710678118654752440084436210485);
+  constexpr double c = 0.70710678118654752440084436210485;
   if constexpr (K == 0) return v;
-  else if constexpr (K == 1) return cmul(v, T{c, DIR * c});
+  else if constexpr (K == 1) return cmul_k(v, c, DIR * c);
   else if constexpr (K == 2) return mul_di<DIR>(v);
-  else return cmul(v, T{-c, DIR * c});
+  else return cmul_k(v, -c, DIR * c);
 }
 
 template <int DIR, typename T> __device__ __forceinline__ void dft8(T* x) {
@@ -84,18 +115,17 @@ template <int DIR, typename T> __device__ __forceinline__ void dft8(T* x) {
 
 template <int DIR, typename T> __device__ __forceinline__ T tw16(T v, int m) {
   // exp(DIR*2πi*m/16) for the m values the radix-16 kernel uses (0..9)
-  using R = decltype(v.x);
-  const R c1 = R(0.92387953251128675612818318939679);  // cos(π/8)
-  const R s1 = R(0.38268343236508977172845998403040);  // sin(π/8)
-  const R c2 = R(0.70710678118654752440084436210485);
+  constexpr double c1 = 0.92387953251128675612818318939679;  // cos(π/8)
+  constexpr double s1 = 0.38268343236508977172845998403040;  // sin(π/8)
+  constexpr double c2 = 0.70710678118654752440084436210485;
   switch (m) {
     case 0: return v;
-    case 1: return cmul(v, T{c1, DIR * s1});
-    case 2: return cmul(v, T{c2, DIR * c2});
-    case 3: return cmul(v, T{s1, DIR * c1});
+    case 1: return cmul_k(v, c1, DIR * s1);
+    case 2: return cmul_k(v, c2, DIR * c2);
+    case 3: return cmul_k(v, s1, DIR * c1);
     case 4: return mul_di<DIR>(v);
-    case 6: return cmul(v, T{-c2, DIR * c2});
-    case 9: return cmul(v, T{-c1, -DIR * s1});
+    case 6: return cmul_k(v, -c2, DIR * c2);
+    case 9: return cmul_k(v, -c1, -DIR * s1);
     default: return v;
   }
 }
@@ -182,7 +212,8 @@ template <int N, int ALT = 0> struct PlanOf {
 
 // ---------------------------------------------------------------- one Stockham transform
 // v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
-// thread's transform through addr(p). `roots[j] = exp(-2πi j/N)` (N entries).
+// thread's transform through addr(p). `roots[j] = exp(-2πi j/N)` (N entries; in FP32 followed
+// by N entries of remainders, roots[N + j] = exp(-2πi j/N) - roots[j] rounded to FP32).
 template <int N, int ALT, int DIR, int S, typename T, typename Addr>
 __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                           int tw_mode) {
@@ -201,7 +232,18 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
     if constexpr (NS > 1) {
-      if (tw_mode == 0) {
+      if constexpr (sizeof(x[0].x) == 4) {
+        // FP32: every twiddle from the table as hi + lo (roots[N + j] = remainder of roots[j])
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          T wh = roots[r * k * TWS], wl = roots[N + r * k * TWS];
+          if constexpr (DIR > 0) {
+            wh = cconj(wh);
+            wl = cconj(wl);
+          }
+          x[r] = cadd(cmul(x[r], wl), cmul(x[r], wh));
+        }
+      } else if (tw_mode == 0) {
         // every twiddle from the FP64-built root table
 #pragma unroll
         for (int r = 1; r < R; ++r) {
@@ -266,7 +308,8 @@ __device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr
   }
 }
 
-// tw_mode 0: all twiddles from the table; 1: two loads and products per butterfly.
+// tw_mode 0: all twiddles from the table; 1: two loads and products per butterfly (FP64 only;
+// FP32 always reads hi + lo twiddles from the table).
 template <int N, int ALT, int DIR, typename T, typename Addr>
 __device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                     int tw_mode) {

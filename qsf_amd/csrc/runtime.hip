@@ -158,6 +158,7 @@ struct Grid {
   int rep = QSF_REP_X;
   int slot = 0;
   static constexpr int kSlots = 48;
+  static constexpr size_t kTabBytes = 16;   // phase tables are complex FP64 in both precisions
 
   Grid(qsf_ctx* c, const qsf_grid_desc& desc) : ctx(c), d(desc) {
     D = d.dim;
@@ -213,7 +214,7 @@ struct Grid {
     psi.ensure(m * cb);
     QSF_HIP(hipMemsetAsync(psi.p, 0, m * cb, ctx->stream));
     int64_t maxn = *std::max_element(n, n + 3);
-    tabs.ensure((size_t)kSlots * 3 * maxn * cb);
+    tabs.ensure((size_t)kSlots * 3 * maxn * kTabBytes);
     red.ensure(64 * sizeof(double));
     redscratch.ensure(256 * 16 * sizeof(double));
     build_static_tables();
@@ -225,7 +226,7 @@ struct Grid {
     auto it = roots.find(len);
     if (it != roots.end()) return it->second->p;
     auto b = std::make_unique<DBuf>();
-    b->ensure(len * cb);
+    b->ensure(len * cb * (prec == QSF_FP64 ? 1 : 2));   // FP32: hi then lo remainders
     launch_roots(prec, b->p, len, st());
     const void* p = b->p;
     roots[len] = std::move(b);
@@ -293,13 +294,13 @@ struct Grid {
   // ---- per-substep tables (ring of slots, each with D per-axis tables) ----
   void* next_slot() {
     int64_t maxn = *std::max_element(n, n + 3);
-    void* p = static_cast<char*>(tabs.p) + (size_t)slot * 3 * maxn * cb;
+    void* p = static_cast<char*>(tabs.p) + (size_t)slot * 3 * maxn * kTabBytes;
     slot = (slot + 1) % kSlots;
     return p;
   }
   void* slot_axis(void* s, int a) const {
     int64_t maxn = *std::max_element(n, n + 3);
-    return static_cast<char*>(s) + (size_t)a * maxn * cb;
+    return static_cast<char*>(s) + (size_t)a * maxn * kTabBytes;
   }
 
   bool reduced() const { return d.potential == QSF_POT_EBERLY || d.potential == QSF_POT_ECKHARDT_SACHA; }
@@ -330,7 +331,7 @@ struct Grid {
     pa.xmin = xmin[a];
     pa.dx = dx[a];
     auto b = std::make_unique<DBuf>();
-    b->ensure((pa.kind == PAIR_DIFF ? pa.n : pa.n * pa.n) * cb);
+    b->ensure((pa.kind == PAIR_DIFF ? pa.n : pa.n * pa.n) * kTabBytes);
     pa.out = b->p;
     launch_pair_table(prec, pa, st());
     const void* p = b->p;
@@ -1461,14 +1462,14 @@ struct Batch {
       for (size_t k = 0; k < G.members.size(); ++k) {
         Solve& so = solves[G.members[k]];
         Grid& g = *so.g;
-        so.ptab.ensure(2 * std::max(g.n[0], g.n[1]) * g.cb);
+        so.ptab.ensure(2 * std::max(g.n[0], g.n[1]) * Grid::kTabBytes);
         so.partials.ensure(std::max<int64_t>(G.tiles_row, 1) * 3 * sizeof(double));
         so.rec.ensure((so.total + 1) * (1 + 16) * sizeof(double));
         so.fluxint.ensure(16 * sizeof(double));
         QSF_HIP(hipMemset(so.fluxint.p, 0, 16 * sizeof(double)));
         if (p.nborders > 0) so.gathered.ensure(std::max<int64_t>(L.gather->nvals, 1) * g.cb);
         char* pt = static_cast<char*>(so.ptab.p);
-        const int64_t half = std::max(g.n[0], g.n[1]) * g.cb;
+        const int64_t half = std::max(g.n[0], g.n[1]) * Grid::kTabBytes;
         be[k] = BatchEntry{g.psi.p, {pt, pt + half, nullptr}, so.partials.as<double>(),
                            p.nborders > 0 ? so.gathered.p : nullptr, so.total};
         BatchTableArgs& t = ta[G.members[k]];

@@ -25,7 +25,11 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, 0>::THREADS))
   const int tid = threadIdx.x;
   const int t = ROW ? (tid % TT) : (tid / W);
   const int w = ROW ? (tid / TT) : (tid % W);
-  const int64_t b = blockIdx.x;
+  // XCD-aware tile order (blocks go round-robin over the 8 XCDs): consecutive column tiles
+  // share HBM lines, so they run on one XCD and its L2 serves the neighbours' partial lines
+  int64_t b = blockIdx.x;
+  const int64_t nb = gridDim.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
   // 2D grid (n0, n1): f = axis; row mode transforms along axis 1 (o = row i0), column mode
   // along axis 0 (c = column i1)
   int64_t o, c;
@@ -83,7 +87,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, 0>::THREADS))
   }
 
   T* dst = reinterpret_cast<T*>(p.dst);
-  XEval<T, N> xa;
+  XEval<N> xa;
   if (p.has_xa) xa.init(p.xa, ROW ? 1 : 0, i0, i1, 0);
   T* eout = reinterpret_cast<T*>(p.edge_out);
   int64_t mo = 0;
@@ -99,7 +103,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, 0>::THREADS))
     const int64_t i = t + TT * q;
     if (!valid) continue;
     T d = dst[base + i * step];
-    if (p.has_xa) d = cmul(d, xa.at((int)i));
+    if (p.has_xa) d = capply(d, xa.at((int)i));
     T r = T{d.x + v[q].x, d.y + v[q].y};
     if (eout) {
       // this receiver's edge nodes along the axis the next group frees

@@ -47,7 +47,7 @@ def traj1d_case():
     f = [T.field_desc(T.AXIS_X, 0.0534, 0.057, 4)]
     p = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_LENGTH,
                     outputs=T.OUT_NORM | T.OUT_EPOT | T.OUT_EKIN | T.OUT_ETOT | T.OUT_FLUX,
-                    borders=[(T.BORDER_BOX, 100)])
+                    borders=[(T.BORDER_BOX, 12)])
     return d, p, f, psi0
 
 
