@@ -141,6 +141,38 @@ def run_c4(steps, warmup):
                 sides={int(n): int(sum(1 for d in descs32 if d.n[0] == n)) for n in C4_SIDES})
 
 
+def run_c5(count, streams=4):
+    """The serving loop on a seeded Poisson stream (mean 4 requests/s) of mixed solves: 60% 1D
+    N=4096, 35% 2D sides 512-2048, 5% 3D 256^3, FP32-mixed, every 10th with an FP64 twin."""
+    from qsf_amd import hip, serve
+    reqs = serve.poisson_stream(count, rate=4.0, seed=5)
+    # one autotuning pass per distinct grid shape and precision; every stream applies its table
+    ctx = hip.Context(0)
+    seen = set()
+    for r in reqs:
+        key = (r.desc.dim, tuple(r.desc.n), r.desc.precision)
+        if key in seen:
+            continue
+        seen.add(key)
+        gs = hip.GridSet(ctx, r.desc)
+        gs[0].set_psi(r.psi0)
+        gs[0].autotune(3)
+        gs.close()
+    table = ctx.tune_table()
+    ctx.close()
+    progress(f"autotuned {len(seen)} shapes; serving {len(reqs)} requests")
+    srv = serve.Server(streams=streams, tuning=table)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, log=progress)
+    wall = time.perf_counter() - t0
+    srv.close()
+    summ = serve.summarize(res, wall)
+    summ["wall_s"] = wall
+    summ["ticks"] = srv.ticks
+    summ["kinds"] = {k: sum(1 for r in reqs if r.kind == k and r.shadow_of < 0) for k in ("1d", "2d", "3d")}
+    return summ
+
+
 def initial_state(m, seed=0):
     rng = np.random.default_rng(seed)
     return (rng.normal(size=m) + 1j * rng.normal(size=m)) * 1e-3
@@ -265,8 +297,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "copy"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--requests", type=int, default=40, help="c5: number of requests in the stream")
     ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -276,6 +309,23 @@ def main():
 
     if args.workload == "copy":
         print(json.dumps(copy_calibration(args.steps)))
+        return
+    if args.workload == "c5":
+        r = run_c5(args.requests)
+        out = {"metric": "grid-point-steps/s", "value": round(r["throughput_grid_point_steps_per_s"], 1),
+               "unit": "grid-point-steps/s", "n_gpus": 1, "steps": None, "warmup": 0,
+               "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "c64 (f32, f64 phases and reductions); FP64 twins for verification",
+               "data": "synthetic",
+               "config": {"workload": f"serving: {args.requests} requests, Poisson 4/s (seed 5), mix 60% 1D "
+                                      "N=4096 / 35% 2D 512-2048 / 5% 3D 256^3, full pulses, continuous "
+                                      "batching over 4 streams, admission by HBM footprint (288 GB)",
+                          "requests_per_kind": r["kinds"]},
+               "p50_s": r["p50_s"], "p99_s": r["p99_s"], "requests_per_s": r["requests_per_s"],
+               "wall_s": r["wall_s"], "ticks": r["ticks"],
+               "verification": {"pairs": r["verification_pairs"], "max_yield_rel_err": r["max_yield_rel_err"],
+                                "checks": r["checks"]}}
+        print(json.dumps(out))
         return
     if args.workload == "c4":
         r = run_c4(args.steps, args.warmup)

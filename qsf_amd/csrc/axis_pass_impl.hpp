@@ -23,6 +23,18 @@ template <typename Real> struct Tr {
   using T = typename cvec<Real>::T;
 };
 
+// Row-mode exchange layout. Stockham stage 0 writes element j*R + r from lane j (stride R);
+// unpadded, 16 lanes of one write pass then hit two 4-byte-bank groups (FP64, R = 8: 128 B
+// apart) — an 8-way conflict. One pad element per 256 B of line (16 complex FP64 or 32 complex
+// FP32, one full sweep of the 64 banks) spreads those lanes over distinct banks.
+template <typename T> struct RowAddr {
+  static constexpr int SH = sizeof(T) == 16 ? 4 : 5;
+  int base;
+  __device__ __forceinline__ int operator()(int p) const { return base + p + (p >> SH); }
+};
+// elements of LDS one padded line occupies
+template <typename T, int N> constexpr int row_stride() { return N + (N >> RowAddr<T>::SH); }
+
 // Kernel variant V (0..5): ALT = V / 3 selects the elements-per-thread plan, V % 3 scales the
 // tile width W (x1, x1/2, x2) within the LDS and block-size limits.
 constexpr int kVariants = 6;
@@ -34,19 +46,25 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
   using PL = PlanOf<N, var_alt(V)>;
   static constexpr int T = PL::T;
   static constexpr int CB = (int)sizeof(typename cvec<Real>::T);
-  static constexpr int lds_cap(int w) { return w * N * CB; }
+  // LDS bytes a block allocates (padded exchange layout)
+  static constexpr int lds_cap(int w) {
+    using T = typename cvec<Real>::T;
+    return (ROW ? w * row_stride<T, N>() : w * N + ((w * N) >> RowAddr<T>::SH)) * CB;
+  }
+  // the data the tile holds, which the occupancy budgets below are stated for
+  static constexpr int lds_data(int w) { return w * N * CB; }
   static constexpr int base() {
     if (ROW) {
       // at least 128 threads per block, at most 32 KiB of LDS: several blocks per CU overlap
       // their load, transform and store phases
       int w = 128 / T;
       if (w < 1) w = 1;
-      while (w > 1 && lds_cap(w) > 32768) w /= 2;
+      while (w > 1 && lds_data(w) > 32768) w /= 2;
       return w;
     }
     // columns: W adjacent columns per block; 64 KiB LDS keeps two blocks per CU
     int w = 16;
-    while (w > 1 && (lds_cap(w) > 65536 || w * T > 1024)) w /= 2;
+    while (w > 1 && (lds_data(w) > 65536 || w * T > 1024)) w /= 2;
     return w;
   }
   static constexpr int pick() {
@@ -59,13 +77,15 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
   static constexpr int THREADS = W * T;
 };
 
-template <typename T> struct RowAddr {
-  int base;
-  __device__ __forceinline__ int operator()(int p) const { return base + p; }
-};
-template <int W> struct ColAddr {
+// Column-mode exchange layout: element p of column w at p*W + w, padded like the rows (with
+// narrow tiles, W = 2 for 2048-point FP64 columns, stage-0 writes of 16 lanes otherwise share
+// one 32-byte bank window).
+template <typename T, int W> struct ColAddr {
   int w;
-  __device__ __forceinline__ int operator()(int p) const { return p * W + w; }
+  __device__ __forceinline__ int operator()(int p) const {
+    const int a = p * W + w;
+    return a + (a >> RowAddr<T>::SH);
+  }
 };
 
 template <typename T>
@@ -294,10 +314,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
   auto run_fft = [&](auto dir_tag) {
     constexpr int DIR = decltype(dir_tag)::value;
     if constexpr (ROW) {
-      RowAddr<T> a{w * N};
+      RowAddr<T> a{w * row_stride<T, N>()};
       fft<N, ALT, DIR>(v, t, lds, a, roots, p.tw_mode);
     } else {
-      ColAddr<W> a{w};
+      ColAddr<T, W> a{w};
       fft<N, ALT, DIR>(v, t, lds, a, roots, p.tw_mode);
     }
   };
@@ -466,7 +486,7 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
   int64_t tiles;
   if (ROW) tiles = (p.outer + G::W - 1) / G::W;
   else tiles = p.outer * ((p.inner + G::W - 1) / G::W);
-  size_t lds = (size_t)G::W * N * sizeof(typename cvec<Real>::T);
+  size_t lds = (size_t)G::lds_cap(G::W);
   if (lds < 16 * 3 * sizeof(double)) lds = 16 * 3 * sizeof(double);
   static bool attr_set = false;
   if (!attr_set) {

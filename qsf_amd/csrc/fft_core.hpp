@@ -31,8 +31,18 @@ template <int DIR, typename T> __device__ __forceinline__ T mul_di(T a) {
 }
 
 // Products by constants and tables that are the same at every step. In FP32 a once-rounded
-// constant would bias every transform the same way (a norm drift linear in the step count), so
-// the FP32 forms carry the rounding remainder: c = c_hi + c_lo, v*c = v*c_hi + v*c_lo.
+// constant biases every transform the same way (fl(sqrt(1/2)) is 1.7e-8 short, so each
+// rotation shrinks the amplitude: a norm drift linear in the step count). The FP32 forms carry
+// the rounding remainder, c = c_hi + c_lo, and add v*c_lo inside the fused multiply-add of
+// v*c_hi, so the correction (below half an ulp) enters before the single rounding instead of
+// being rounded away afterwards.
+__device__ __forceinline__ float2 cmul_comp(float2 v, float2 h, float2 l) {
+  const float cr = v.x * l.x - v.y * l.y;   // small: v * c_lo
+  const float ci = v.x * l.y + v.y * l.x;
+  const float re = __builtin_fmaf(v.x, h.x, __builtin_fmaf(-v.y, h.y, cr));
+  const float im = __builtin_fmaf(v.x, h.y, __builtin_fmaf(v.y, h.x, ci));
+  return float2{re, im};
+}
 template <typename T> __device__ __forceinline__ T cmul_k(T v, double cr, double ci) {
   using R = decltype(v.x);
   if constexpr (sizeof(R) == 8) {
@@ -40,7 +50,7 @@ template <typename T> __device__ __forceinline__ T cmul_k(T v, double cr, double
   } else {
     const float hr = (float)cr, hi = (float)ci;
     const float lr = (float)(cr - (double)hr), li = (float)(ci - (double)hi);
-    return cadd(cmul(v, T{lr, li}), cmul(v, T{hr, hi}));
+    return cmul_comp(v, float2{hr, hi}, float2{lr, li});
   }
 }
 template <typename T> __device__ __forceinline__ T cscale_k(T v, double c) {
@@ -49,7 +59,7 @@ template <typename T> __device__ __forceinline__ T cscale_k(T v, double c) {
     return T{v.x * c, v.y * c};
   } else {
     const float h = (float)c, l = (float)(c - (double)h);
-    return T{v.x * h + v.x * l, v.y * h + v.y * l};
+    return T{__builtin_fmaf(v.x, h, v.x * l), __builtin_fmaf(v.y, h, v.y * l)};
   }
 }
 // state times an FP64 phase factor: FP64 product, one rounding to the state type
@@ -241,7 +251,7 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
             wh = cconj(wh);
             wl = cconj(wl);
           }
-          x[r] = cadd(cmul(x[r], wl), cmul(x[r], wh));
+          x[r] = cmul_comp(x[r], wh, wl);
         }
       } else if (tw_mode == 0) {
         // every twiddle from the FP64-built root table

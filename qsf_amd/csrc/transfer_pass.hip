@@ -79,10 +79,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, 0>::THREADS))
 
   const T* roots = reinterpret_cast<const T*>(p.roots);
   if constexpr (ROW) {
-    RowAddr<T> a{w * N};
+    RowAddr<T> a{w * row_stride<T, N>()};
     fft<N, 0, -1>(v, t, lds, a, roots, p.tw_mode);
   } else {
-    ColAddr<W> a{w};
+    ColAddr<T, W> a{w};
     fft<N, 0, -1>(v, t, lds, a, roots, p.tw_mode);
   }
 
@@ -141,7 +141,7 @@ template <typename Real, int N, bool ROW>
 static int launch_tr(const TransferParams& p, hipStream_t s) {
   using G = Geo<Real, N, ROW, 0>;
   int64_t tiles = ROW ? (p.n0 + G::W - 1) / G::W : (p.n1 + G::W - 1) / G::W;
-  size_t lds = (size_t)G::W * N * sizeof(typename cvec<Real>::T);
+  size_t lds = (size_t)G::lds_cap(G::W);
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&transfer_kernel<Real, N, ROW>),
