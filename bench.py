@@ -182,7 +182,7 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_device(desc, pdesc, fields, steps, warmup, profile=True):
+def run_device(desc, pdesc, fields, steps, warmup, profile=True, keep_state=False):
     from qsf_amd import hip
     progress(f"device run: precision {desc.precision}, {steps} steps after {warmup} warm-up")
     ctx = hip.Context(0)
@@ -208,14 +208,17 @@ def run_device(desc, pdesc, fields, steps, warmup, profile=True):
         ctx.profile_begin()
         prop.run_steps(steps)
         kernels = ctx.profile_end()
+    else:
+        prop.run_steps(steps)   # same step count as a profiled run, so the two can be compared
     rec = prop.records()
+    psi = gset[0].psi() if keep_state else None
     m_main = gset[0].m
     m_all = sum(g.m for g in gset.grids)
     prop.close()
     gset.close()
     ctx.close()
     return dict(done=done, wall=wall, dev_ms=dev_ms, m_main=m_main, m_all=m_all, kernels=kernels,
-                last=rec[-1].tolist() if len(rec) else None)
+                records=rec, psi=psi)
 
 
 def measured_traffic(kernel):
@@ -355,7 +358,7 @@ def main():
                     "dt=0.05, 2-cycle 800 nm, CAP 32 nodes, BOX<224> flux + norm every step")
         pts_bytes = 128
 
-    r = run_device(desc, pdesc, fields, args.steps, args.warmup)
+    r = run_device(desc, pdesc, fields, args.steps, args.warmup, keep_state=True)
     ms_per_step = r["wall"] / r["done"] * 1e3
     value = r["m_main"] * r["done"] / r["wall"]
     out = {
@@ -373,9 +376,20 @@ def main():
     }
     if not args.no_variants:
         if args.workload == "c2":
-            m = run_device(c2_desc(T.FP32_MIXED), pdesc, fields, args.steps, args.warmup, profile=False)
-            out["fp32_mixed"] = {"value": round(m["m_main"] * m["done"] / m["wall"], 1),
-                                 "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4)}
+            m = run_device(c2_desc(T.FP32_MIXED), pdesc, fields, args.steps, args.warmup, profile=False,
+                           keep_state=True)
+            a, b = m["records"], r["records"]
+            nb = len(T.ZOA_FLUX_2D)
+            flux_err = np.abs(a[:, T.REC_FLUX0:T.REC_FLUX0 + nb] - b[:, T.REC_FLUX0:T.REC_FLUX0 + nb]).max() / \
+                max(np.abs(b[:, T.REC_FLUX0:T.REC_FLUX0 + nb]).max(), 1e-300)
+            out["fp32_mixed"] = {
+                "value": round(m["m_main"] * m["done"] / m["wall"], 1),
+                "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4),
+                "vs_fp64_same_steps": {
+                    "steps": int(b[-1, T.REC_STEP]),
+                    "norm_rel_err": float(abs(a[-1, T.REC_NORM] - b[-1, T.REC_NORM]) / abs(b[-1, T.REC_NORM])),
+                    "flux_max_rel_err": float(flux_err),
+                    "psi_l2_rel_err": float(np.linalg.norm(m["psi"] - r["psi"]) / np.linalg.norm(r["psi"]))}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(desc, pdesc, fields)
     print(json.dumps(out))

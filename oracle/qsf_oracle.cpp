@@ -500,9 +500,12 @@ struct Grid {
   double potentialX(const double* c) const { return many ? partial(freeMask, c) : full(c); }
 
   // ---- absorbers ----
-  double cos18_node(ind node) const {  // cos^(1/8) edge profile over nCAP nodes (new)
+  // cos^(1/8) edge profile over nCAP nodes (new): cos(π/2 · node/nCAP) written as the sine of
+  // the distance from the edge, sin(π/2 · (nCAP − node)/nCAP), which is exactly 0 at the edge
+  // node; the cosine form can round to a tiny negative value there and pow(·, 1/8) is then NaN.
+  double cos18_node(ind node) const {
     if (node <= 0) return 1.0;
-    return std::pow(std::cos(0.5 * pi * double(node) / double(d.nCAP)), 0.125);
+    return std::pow(std::sin(0.5 * pi * double(d.nCAP - node) / double(d.nCAP)), 0.125);
   }
   // CAP::mask (absorbers.h:48-54) / cos^(1/8) product over axes
   double mask(int k, const ind* nd) const {

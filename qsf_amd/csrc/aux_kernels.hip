@@ -41,7 +41,8 @@ __global__ void axis_tables_kernel(AxisTableBatch batch) {
         const double s4 = (double)(node * (node * (node * node)));
         f = exp(-a.delta * s4 * a.eta);                  // absorbers.h:40-46, one axis
       } else {
-        const double c = cos(0.5 * M_PI * (double)node / (double)a.nCAP);
+        // cos(π/2 · node/nCAP) as the sine of the distance from the edge (never negative)
+        const double c = sin(0.5 * M_PI * (double)(a.nCAP - node) / (double)a.nCAP);
         f = pow(pow(c, 0.125), a.delta / a.tau);         // cos^(1/8) per tau of time
       }
       re *= f;

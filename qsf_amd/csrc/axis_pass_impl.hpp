@@ -280,6 +280,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
     i0 = (int)(o / p.n[1]);
     i1 = (int)(o % p.n[1]);
   }
+  if (!valid) i0 = i1 = i2 = 0;   // idle threads of a partial tile read in-range table entries
   auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
 
   const T* roots = reinterpret_cast<const T*>(p.roots);
@@ -486,14 +487,19 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
   int64_t tiles;
   if (ROW) tiles = (p.outer + G::W - 1) / G::W;
   else tiles = p.outer * ((p.inner + G::W - 1) / G::W);
+  const size_t red_bytes = 16 * 3 * sizeof(double);   // block reduction: 3 sums per wave
   size_t lds = (size_t)G::lds_cap(G::W);
-  if (lds < 16 * 3 * sizeof(double)) lds = 16 * 3 * sizeof(double);
+  if (lds < red_bytes) lds = red_bytes;
   static bool attr_set = false;
   if (!attr_set) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     attr_set = true;
   }
+  // a pass without a transform or a flux gather (pointwise phases of a region whose axes all
+  // stay in P) uses LDS only for the reduction: more blocks fit on a CU
+  const bool line = (p.flags & (F_INV1 | F_FWD | F_INV2)) || (ROW && (p.flags & F_OBS) && p.obs.gather_rowptr);
+  if (!line) lds = red_bytes;
   const unsigned ny = p.batch ? (unsigned)p.nbatch : 1u;
   hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V>), dim3((unsigned)tiles, ny), dim3(G::THREADS), lds, s, p);
   return (int)tiles;

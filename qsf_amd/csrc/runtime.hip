@@ -238,7 +238,9 @@ struct Grid {
     if (d.absorber == QSF_ABS_COS18) {
       double r = 1.0;
       for (int i = 0; i < k; ++i)
-        r = r * (nodes[i] > 0 ? std::pow(std::cos(0.5 * phys::kPi * double(nodes[i]) / double(d.nCAP)), 0.125) : 1.0);
+        // cos(π/2 · node/nCAP) as sin(π/2 · (nCAP − node)/nCAP): exactly 0 at the edge node
+        r = r * (nodes[i] > 0 ? std::pow(std::sin(0.5 * phys::kPi * double(d.nCAP - nodes[i]) / double(d.nCAP)), 0.125)
+                              : 1.0);
       return r;
     }
     bool allneg = true;
@@ -860,6 +862,16 @@ struct Set {
   // Edge buffers of the fused transfer: main region edge rows/columns, region X edge columns
   // and region Y edge rows (the sources of the XY region).
   DBuf er_main, ec_main, ec_x, er_y;
+
+  // the pending phase tables of every region grid in one batched launch (24 per launch)
+  void build_tables() {
+    std::vector<AxisTableArgs> all;
+    for (auto& gp : g) {
+      all.insert(all.end(), gp->pending_tables.begin(), gp->pending_tables.end());
+      gp->pending_tables.clear();
+    }
+    if (!all.empty()) launch_axis_tables(g[0]->prec, all.data(), (int)all.size(), ctx->stream);
+  }
   void ensure_edges() {
     const Grid& m0 = *g[0];
     const size_t bytes_r = 2 * m0.d.nCAP * m0.n[1] * m0.cb, bytes_c = m0.n[0] * 2 * m0.d.nCAP * m0.cb;
@@ -1089,23 +1101,40 @@ struct Prop {
   }
 
   void exec_fused(const std::vector<SubOp>& ops) {
-    for (size_t r = 0; r < set->g.size(); ++r) {
+    // every phase table of this step first (each grid's own ring of slots), so the tables of
+    // all region grids build in one batched launch instead of one launch per pass
+    struct Sub { XFactor fb; const void* pf[3]; };
+    const size_t R = set->g.size();
+    std::vector<std::vector<Sub>> subs(R);
+    std::vector<XFactor> last(R);
+    for (size_t r = 0; r < R; ++r) {
       Grid& g = *set->g[r];
-      if (r == 1 && pend_post) run_receives();
-      const bool obs_here = (r == 0) && pend_obs;
       for (size_t SI = 0; SI < ops.size(); ++SI) {
         const SubOp& o = ops[SI];
         if (o.rep != QSF_REP_P) continue;
         const SubOp& xb = ops[SI - 1];
-        XFactor fb = g.x_factor(pd.mode, xb.delta, xb.C, couples() == QSF_REP_X);
-        const void* pf[3];
-        g.p_factor(pd.mode, o.delta, o.C, couples() == QSF_REP_P, true, pf);
-        const bool first = (SI == 1);
-        run_p_substep(g, r, fb, pf, first && obs_here, first && r == 0 && pend_post);
+        Sub sb;
+        sb.fb = g.x_factor(pd.mode, xb.delta, xb.C, couples() == QSF_REP_X);
+        g.p_factor(pd.mode, o.delta, o.C, couples() == QSF_REP_P, true, sb.pf);
+        subs[r].push_back(sb);
       }
       const SubOp& xl = ops.back();
+      last[r] = g.x_factor(pd.mode, xl.delta, xl.C, couples() == QSF_REP_X);
+    }
+    set->build_tables();
+    for (size_t r = 0; r < R; ++r) {
+      Grid& g = *set->g[r];
+      if (r == 1 && pend_post) run_receives();
+      const bool obs_here = (r == 0) && pend_obs;
+      size_t k = 0;
+      for (size_t SI = 0; SI < ops.size(); ++SI) {
+        if (ops[SI].rep != QSF_REP_P) continue;
+        const bool first = (SI == 1);
+        const Sub& sb = subs[r][k++];
+        run_p_substep(g, r, sb.fb, sb.pf, first && obs_here, first && r == 0 && pend_post);
+      }
       pend[r].on = true;
-      pend[r].f = g.x_factor(pd.mode, xl.delta, xl.C, couples() == QSF_REP_X);
+      pend[r].f = last[r];
       deferred[r] = true;
     }
     if (pend_obs) pend_obs = false;

@@ -50,13 +50,12 @@ def test_mask_fixture_reproduced_bitwise_and_its_invariants():
             assert np.all(m[c:n - c, c:n - c] == 1.0)
         # the last region (both axes free) is not masked (multigrid.h:617-625)
         assert np.all(tab[3] == 1.0)
-        # the main region's corner node, both axes at the edge: CAP exp(-81 * 2);
-        # cos^(1/8)(pi/2)^2 (cos(pi/2) is 6.1e-17 in FP64, not 0)
+        # the main region's corner node, both axes at the edge: CAP exp(-81 * 2); cos^(1/8) 0
         corner = tab[0][0, 0]
         if a == T.ABS_CAP:
             assert np.isclose(corner, np.exp(-162.0), rtol=1e-12)
         else:
-            assert np.isclose(corner, (np.cos(0.5 * np.pi) ** 0.125) ** 2, rtol=1e-12)
+            assert corner == 0.0
 
 
 def test_border_fixture_reproduced_and_normalised():

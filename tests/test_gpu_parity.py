@@ -355,3 +355,29 @@ def test_batch_matches_single_solves(ctx, use_graph):
         assert np.allclose(rb[:, 2], ro[:, T.REC_NORM], rtol=1e-11)
         assert np.allclose(rb[:, 3:], ro[:, T.REC_FLUX0:], rtol=1e-8, atol=1e-12 * np.abs(ro[:, T.REC_FLUX0:]).max())
     b.close()
+
+
+def test_cos18_edge_node_stays_finite(ctx):
+    """cos^(1/8) with a width whose π/2·node/nCAP rounds past π/2 at the edge node (nCAP=13):
+    the mask there is exactly 0 on both sides and the propagation stays finite."""
+    n, dx = 128, 0.2
+    d = T.grid_desc(2, n, dx, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                    absorber=T.ABS_COS18, nCAP=13, absorber_tau=0.02)
+    x = -0.5 * dx * (n - 1) + dx * np.arange(n)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    psi0 = (np.exp(-0.5 * (X * X + Y * Y)) + 0j).ravel()
+    f = [T.field_desc(T.AXIS_XY, 0.1, 0.5, 1)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.02, gauge=T.GAUGE_VELOCITY, outputs=T.OUT_NORM | T.OUT_FLUX,
+                        borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+    dev, ref, pd, po, rd, ro = run_both(ctx, d, pdesc, f, psi0, 20)
+    assert np.all(np.isfinite(rd[:, T.REC_NORM])) and np.all(np.isfinite(dev[0].psi()))
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+    # the mask of the main region is 0 on the edge rows and columns
+    md = T.grid_desc(2, n, dx, absorber=T.ABS_COS18, nCAP=13, absorber_tau=0.02, multigrid=1)
+    from qsf_amd import hip
+    gs = hip.GridSet(ctx, md)
+    gs[0].set_psi(np.ones(n * n, np.complex128))
+    gs[0].mask_region()
+    m = gs[0].psi().real.reshape(n, n)
+    assert np.all(m[0, :] == 0.0) and np.all(m[:, -1] == 0.0) and np.isfinite(m).all()
+    gs.close()

@@ -123,7 +123,15 @@ def test_cos18_closed_form():
     for dist in range(0, 14):
         node = w - dist
         want = np.cos(0.5 * np.pi * node / w) ** 0.125 if node > 0 else 1.0
-        assert np.isclose(g.absorber_mask([-dist]), want, rtol=1e-14)
+        if node == w:
+            want = 0.0                     # the edge node: cos(π/2) is 0
+        assert np.isclose(g.absorber_mask([-dist]), want, rtol=1e-14, atol=1e-300)
+    # widths whose rounded π/2·node/nCAP lands past π/2 still give a mask in [0, 1], never NaN
+    for w2 in range(1, 64):
+        g2 = O.GridSet(T.grid_desc(1, 256, 0.1, absorber=T.ABS_COS18, nCAP=w2, absorber_tau=tau))[0]
+        vals = [g2.absorber_mask([-d]) for d in range(0, w2 + 1)]
+        assert all(0.0 <= v <= 1.0 for v in vals), w2
+        assert vals[0] == 0.0
     # two half-substeps apply exactly one full mask per tau
     m = g.absorber_mask([-2])
     assert np.isclose(g.absorber_absorb(tau / 2, [-2]) ** 2, m, rtol=1e-14)
