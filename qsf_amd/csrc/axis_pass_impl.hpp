@@ -46,10 +46,11 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
   using PL = PlanOf<N, var_alt(V)>;
   static constexpr int T = PL::T;
   static constexpr int CB = (int)sizeof(typename cvec<Real>::T);
-  // LDS bytes a block allocates (padded exchange layout)
+  // LDS bytes of a block's transform exchange (padded layout; FP32 moves one real component
+  // at a time)
   static constexpr int lds_cap(int w) {
     using T = typename cvec<Real>::T;
-    return (ROW ? w * row_stride<T, N>() : w * N + ((w * N) >> RowAddr<T>::SH)) * CB;
+    return (ROW ? w * row_stride<T, N>() : w * N + ((w * N) >> RowAddr<T>::SH)) * kExchangeBytes<T>;
   }
   // the data the tile holds, which the occupancy budgets below are stated for
   static constexpr int lds_data(int w) { return w * N * CB; }
@@ -251,6 +252,21 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
   int64_t b = blockIdx.x;
   const int64_t nb = gridDim.x;
   if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  if (ROW && p.axis == 2 && p.n[2] > 1) {
+    // 3D rows (axis 2) read pair-table rows indexed by i1 and by i0 (the (1,2) and (2,0) pairs).
+    // Walking lines with i1 fastest cycles all n rows of the table through L2 every n lines
+    // (512 x 8 KiB at 512^3: re-fetched from HBM, +2 GiB per pass); walking (i0, i1) in 32 x 64
+    // patches keeps the rows the in-flight tiles of one XCD use resident.
+    constexpr int64_t P0 = 32, P1 = 64;
+    const int64_t n0 = p.n[0], n1 = p.n[1];
+    if (n0 % P0 == 0 && n1 % P1 == 0 && P1 % W == 0 && nb * W == n0 * n1) {
+      const int64_t tpp = P0 * (P1 / W);  // tiles per patch
+      const int64_t pidx = b / tpp, r = b % tpp, pc = n1 / P1;
+      const int64_t j0 = (pidx / pc) * P0 + r / (P1 / W);
+      const int64_t j1 = (pidx % pc) * P1 + (r % (P1 / W)) * W;
+      b = (j0 * n1 + j1) / W;  // the tile whose first line is (j0, j1)
+    }
+  }
 
   // tile -> (outer o, inner c) of this thread's transform
   int64_t o, c;
@@ -487,19 +503,23 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
   int64_t tiles;
   if (ROW) tiles = (p.outer + G::W - 1) / G::W;
   else tiles = p.outer * ((p.inner + G::W - 1) / G::W);
-  const size_t red_bytes = 16 * 3 * sizeof(double);   // block reduction: 3 sums per wave
-  size_t lds = (size_t)G::lds_cap(G::W);
-  if (lds < red_bytes) lds = red_bytes;
+  // LDS per launch: the block reduction (3 sums per wave), the transform exchange, and for a
+  // flux gather the complex lines themselves; a pointwise pass (a region whose axes all stay in
+  // P) needs only the reduction, so more blocks fit on a CU
+  const size_t red_bytes = 16 * 3 * sizeof(double);
+  const size_t xchg_bytes = (size_t)G::lds_cap(G::W);
+  const size_t gather_bytes = (size_t)G::W * N * G::CB;
   static bool attr_set = false;
   if (!attr_set) {
+    size_t most = red_bytes > xchg_bytes ? red_bytes : xchg_bytes;
+    if (ROW && gather_bytes > most) most = gather_bytes;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
     attr_set = true;
   }
-  // a pass without a transform or a flux gather (pointwise phases of a region whose axes all
-  // stay in P) uses LDS only for the reduction: more blocks fit on a CU
-  const bool line = (p.flags & (F_INV1 | F_FWD | F_INV2)) || (ROW && (p.flags & F_OBS) && p.obs.gather_rowptr);
-  if (!line) lds = red_bytes;
+  size_t lds = red_bytes;
+  if ((p.flags & (F_INV1 | F_FWD | F_INV2)) && xchg_bytes > lds) lds = xchg_bytes;
+  if (ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && gather_bytes > lds) lds = gather_bytes;
   const unsigned ny = p.batch ? (unsigned)p.nbatch : 1u;
   hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V>), dim3((unsigned)tiles, ny), dim3(G::THREADS), lds, s, p);
   return (int)tiles;

@@ -221,9 +221,15 @@ template <int N, int ALT = 0> struct PlanOf {
 };
 
 // ---------------------------------------------------------------- one Stockham transform
+// FP32 transforms exchange real and imaginary parts in turn (half the LDS per block); FP64
+// transforms exchange whole complex values.
+template <typename T> constexpr bool kSplitExchange = sizeof(T) == 8;
+template <typename T> constexpr int kExchangeBytes = kSplitExchange<T> ? (int)sizeof(T) / 2 : (int)sizeof(T);
+
 // v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
-// thread's transform through addr(p). `roots[j] = exp(-2πi j/N)` (N entries; in FP32 followed
-// by N entries of remainders, roots[N + j] = exp(-2πi j/N) - roots[j] rounded to FP32).
+// thread's transform through addr(p), in units of one exchanged value (kExchangeBytes).
+// roots[j] = exp(-2πi j/N) (N entries; in FP32 followed by N entries of remainders,
+// roots[N + j] = exp(-2πi j/N) - roots[j] rounded to FP32).
 template <int N, int ALT, int DIR, int S, typename T, typename Addr>
 __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                           int tw_mode) {
@@ -292,19 +298,51 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
       }
     }
     dft<R, DIR>(x);
-    if constexpr (LAST) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b + NB * r] = x[r];
-    } else {
-      const int base = (j / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) lds[addr(base + r * NS)] = x[r];
-    }
+    for (int r = 0; r < R; ++r) v[b + NB * r] = x[r];   // slot b + NB*r holds output r
   }
-  if constexpr (!LAST) {
+  if constexpr (!LAST && !kSplitExchange<T>) {
+    // complex exchange: output r of butterfly b goes to position (j/NS)*NS*R + j%NS + r*NS,
+    // j = t + T*b; element t + T*q comes back
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + TT * b;
+      const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[addr(base + r * NS)] = v[b + NB * r];
+    }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = lds[addr(t + TT * q)];
+    __syncthreads();
+  }
+  if constexpr (!LAST && kSplitExchange<T>) {
+    // Exchange in two halves, real parts then imaginary parts: a block needs half the LDS, so
+    // twice as many blocks stay resident (measured: a gain for FP32 passes, a loss for FP64,
+    // whose extra barriers cost more than the occupancy returns).
+    using Re = decltype(v[0].x);
+    Re* lr = reinterpret_cast<Re*>(lds);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + TT * b;
+      const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lr[addr(base + r * NS)] = v[b + NB * r].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q].x = lr[addr(t + TT * q)];   // the outputs' .y stay put
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int j = t + TT * b;
+      const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lr[addr(base + r * NS)] = v[b + NB * r].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q].y = lr[addr(t + TT * q)];
     __syncthreads();
   }
 }

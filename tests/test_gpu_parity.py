@@ -243,9 +243,10 @@ def test_propagation_2d_velocity_gauge_order2(ctx, exec_mode):
     assert np.allclose(rd[:, 7:], ro[:, 7:], rtol=1e-8, atol=1e-12 * np.abs(ro[:, 7:]).max())
 
 
-@pytest.mark.parametrize("exec_mode", [0, 1])
-def test_propagation_3d_length_gauge(ctx, exec_mode):
-    n, dx = 32, 0.3
+@pytest.mark.parametrize("exec_mode,n", [(0, 32), (1, 32), (0, 64)])
+def test_propagation_3d_length_gauge(ctx, exec_mode, n):
+    # n = 64 takes the patched (i0, i1) line order of the 3D row pass; 32 the linear one
+    dx = 0.3
     d = T.grid_desc(3, n, dx, potential=T.POT_ECKHARDT_SACHA, Ncharge=3, Echarge=-1, Nsoft=1, Esoft=1,
                     absorber=T.ABS_CAP, nCAP=6)
     psi0 = crand(n ** 3) * 0.01
