@@ -354,6 +354,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
     const bool shared = doA && doB && xa.npr > 0 && p.xa.pair_kind == p.xb.pair_kind &&
                         p.xa.pair == p.xb.pair && p.xa.pair_mask == p.xb.pair_mask &&
                         p.xa.delta == p.xb.delta && p.xa.mode == p.xb.mode;
+    // both halves are one and the same factor (the velocity gauge's X factor does not change
+    // from step to step, and the host then hands the same tables twice): evaluate it once
+    const bool same = shared && p.xa.axis[0] == p.xb.axis[0] && p.xa.axis[1] == p.xb.axis[1] &&
+                      p.xa.axis[2] == p.xb.axis[2];
     const ObsParams& ob = p.obs;
     double nrm = 0.0, ep = 0.0, ek = 0.0, kc = 0.0;
     const double* ka = nullptr;
@@ -386,7 +390,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int ia = t + TT * q;
-        if (shared) {
+        if (same) {
+          const double2 z = cmul(xa.base_at(ia), xa.pair_at(ia));
+          v[q] = capply(v[q], cmul(z, z));
+        } else if (shared) {
           const double2 pp = xa.pair_at(ia);
           v[q] = capply(v[q], cmul(cmul(xa.base_at(ia), xb.base_at(ia)), cmul(pp, pp)));
         } else if (doA && doB) {
@@ -400,9 +407,12 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int ia = t + TT * q;
-      double2 pp;
+      double2 pp, za;
       if (shared) pp = xa.pair_at(ia);
-      if (doA) v[q] = capply(v[q], shared ? cmul(xa.base_at(ia), pp) : xa.at(ia));
+      if (doA) {
+        za = shared ? cmul(xa.base_at(ia), pp) : xa.at(ia);
+        v[q] = capply(v[q], za);
+      }
       if (obs && valid) {
         const double a2 = (double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y;
         nrm += a2;
@@ -441,7 +451,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
         const double mk = p.mask_tab[off];
         v[q] = T{(Real)((double)v[q].x * mk), (Real)((double)v[q].y * mk)};
       }
-      if (doB) v[q] = capply(v[q], shared ? cmul(xb.base_at(ia), pp) : xb.at(ia));
+      if (doB) v[q] = capply(v[q], same ? za : (shared ? cmul(xb.base_at(ia), pp) : xb.at(ia)));
     }
     if (obs) {
       // block reduction (fixed order: lanes, then waves)

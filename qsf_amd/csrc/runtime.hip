@@ -342,8 +342,27 @@ struct Grid {
   }
 
   // X factor of one X substep: per-axis tables built now + the static pair table.
+  // X factors that do not couple (velocity gauge, no field term in X) depend on (mode, δ)
+  // only: built once into tables of their own, so every step (and both X halves of a pass)
+  // use the same tables and the pass evaluates the factor once per element
+  std::map<std::pair<int64_t, int>, std::pair<std::unique_ptr<DBuf>, XFactor>> xstatic;
+
   XFactor x_factor(int mode, double delta, const double* C, bool coupled) {
-    void* s = next_slot();
+    std::pair<int64_t, int> key{0, mode};
+    if (!coupled) {
+      std::memcpy(&key.first, &delta, 8);
+      auto it = xstatic.find(key);
+      if (it != xstatic.end()) return it->second.second;
+    }
+    void* s = nullptr;
+    std::unique_ptr<DBuf> own;
+    if (coupled) {
+      s = next_slot();
+    } else {
+      own.reset(new DBuf());
+      own->ensure((size_t)3 * *std::max_element(n, n + 3) * kTabBytes);
+      s = own->p;
+    }
     AxisTableArgs args[3];
     for (int a = 0; a < D; ++a) {
       AxisTableArgs& t = args[a];
@@ -397,6 +416,7 @@ struct Grid {
         f.pair_kind = PAIR_FULL;
       }
     }
+    if (!coupled) xstatic.emplace(key, std::make_pair(std::move(own), f));
     return f;
   }
 
