@@ -108,9 +108,13 @@ __global__ void roots_kernel(typename cvec<Real>::T* out, int64_t n) {
   double s, c;
   sincospi(-2.0 * (double)j / (double)n, &s, &c);
   const Real hc = (Real)c, hs = (Real)s;
-  out[j] = typename cvec<Real>::T{hc, hs};
-  // FP32: the rounding remainders follow the n roots (hi + lo carries the FP64 root)
-  if constexpr (sizeof(Real) == 4) out[n + j] = typename cvec<Real>::T{(Real)(c - (double)hc), (Real)(s - (double)hs)};
+  if constexpr (sizeof(Real) == 8) {
+    out[j] = typename cvec<Real>::T{hc, hs};
+  } else {
+    // FP32: {hi, lo} per root, adjacent (hi + lo carries the FP64 root; one 16-byte load)
+    out[2 * j] = typename cvec<Real>::T{hc, hs};
+    out[2 * j + 1] = typename cvec<Real>::T{(Real)(c - (double)hc), (Real)(s - (double)hs)};
+  }
 }
 
 void launch_roots(int precision, void* out, int64_t n, hipStream_t s) {

@@ -228,8 +228,8 @@ template <typename T> constexpr int kExchangeBytes = kSplitExchange<T> ? (int)si
 
 // v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
 // thread's transform through addr(p), in units of one exchanged value (kExchangeBytes).
-// roots[j] = exp(-2πi j/N) (N entries; in FP32 followed by N entries of remainders,
-// roots[N + j] = exp(-2πi j/N) - roots[j] rounded to FP32).
+// roots[j] = exp(-2πi j/N) (N entries; in FP32 each entry is a pair {hi, lo}: roots[2j] the
+// rounded root, roots[2j + 1] its rounding remainder, read as one 16-byte record).
 template <int N, int ALT, int DIR, int S, typename T, typename Addr>
 __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
                                           int tw_mode) {
@@ -249,10 +249,12 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
     if constexpr (NS > 1) {
       if constexpr (sizeof(x[0].x) == 4) {
-        // FP32: every twiddle from the table as hi + lo (roots[N + j] = remainder of roots[j])
+        // FP32: every twiddle from the table as hi + lo, one 16-byte load per root
+        const float4* r4 = reinterpret_cast<const float4*>(roots);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          T wh = roots[r * k * TWS], wl = roots[N + r * k * TWS];
+          const float4 w = r4[r * k * TWS];
+          T wh{w.x, w.y}, wl{w.z, w.w};
           if constexpr (DIR > 0) {
             wh = cconj(wh);
             wl = cconj(wl);

@@ -226,7 +226,7 @@ struct Grid {
     auto it = roots.find(len);
     if (it != roots.end()) return it->second->p;
     auto b = std::make_unique<DBuf>();
-    b->ensure(len * cb * (prec == QSF_FP64 ? 1 : 2));   // FP32: hi then lo remainders
+    b->ensure(len * cb * (prec == QSF_FP64 ? 1 : 2));   // FP32: {hi, lo} record per root
     launch_roots(prec, b->p, len, st());
     const void* p = b->p;
     roots[len] = std::move(b);
