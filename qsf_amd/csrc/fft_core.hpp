@@ -249,17 +249,50 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
     if constexpr (NS > 1) {
       if constexpr (sizeof(x[0].x) == 4) {
-        // FP32: every twiddle from the table as hi + lo, one 16-byte load per root
         const float4* r4 = reinterpret_cast<const float4*>(roots);
+        if (tw_mode == 1) {
+          // FP32, two loads per butterfly: w^k and w^4k as {hi, lo} records taken to FP64,
+          // the other powers by FP64 products, each split back into hi + lo for the
+          // compensated product
+          auto root = [&](int idx) {
+            const float4 q = r4[idx];
+            double2 d{(double)q.x + (double)q.z, (double)q.y + (double)q.w};
+            if constexpr (DIR > 0) d.y = -d.y;
+            return d;
+          };
+          const double2 w1 = root(k * TWS);
+          const double2 w4 = R > 4 ? root(4 * k * TWS) : w1;
+          const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          const float4 w = r4[r * k * TWS];
-          T wh{w.x, w.y}, wl{w.z, w.w};
-          if constexpr (DIR > 0) {
-            wh = cconj(wh);
-            wl = cconj(wl);
+          for (int r = 1; r < R; ++r) {
+            double2 w;
+            if (r == 1) w = w1;
+            else if (r == 2) w = w2;
+            else if (r == 3) w = w3;
+            else if (r == 4) w = w4;
+            else if (r < 8) w = cmul(w4, r == 5 ? w1 : (r == 6 ? w2 : w3));
+            else {
+              const double2 w8 = cmul(w4, w4);
+              const int m = r & 3;
+              const double2 b = r < 12 ? w8 : cmul(w8, w4);
+              w = m == 0 ? b : cmul(b, m == 1 ? w1 : (m == 2 ? w2 : w3));
+            }
+            const float2 wh{(float)w.x, (float)w.y};
+            const float2 wl{(float)(w.x - (double)wh.x), (float)(w.y - (double)wh.y)};
+            x[r] = cmul_comp(x[r], wh, wl);
           }
-          x[r] = cmul_comp(x[r], wh, wl);
+        } else {
+          // FP32: every twiddle from the table as hi + lo, one 16-byte load per root
+#pragma unroll
+          for (int r = 1; r < R; ++r) {
+            const float4 w = r4[r * k * TWS];
+            T wh{w.x, w.y}, wl{w.z, w.w};
+            if constexpr (DIR > 0) {
+              wh = cconj(wh);
+              wl = cconj(wl);
+            }
+            x[r] = cmul_comp(x[r], wh, wl);
+          }
         }
       } else if (tw_mode == 0) {
         // every twiddle from the FP64-built root table

@@ -88,7 +88,9 @@ struct qsf_ctx {
   std::vector<hipEvent_t> spare;
   struct Entry { std::string key; int64_t n; double ms, bytes; };
   std::vector<Entry> entries;
-  int tw_mode[2] = {1, 0};  // FP64: two loads + products; FP32: table loads
+  // twiddles: two loads (w^k, w^4k) and FP64 products per butterfly, in both precisions
+  // (FP32 splits each power into hi + lo; measured faster than one {hi, lo} load per twiddle)
+  int tw_mode[2] = {1, 1};
   bool pair_onfly = false;  // EckhardSacha pair phase per element instead of the n*n table
   // kernel variant per (precision, row mode, axis length, grid size), chosen by the autotuner
   struct TuneKey {
@@ -1727,6 +1729,7 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
   auto* c = new qsf_ctx();
   c->device = device;
   if (const char* e = std::getenv("QSF_TW_MODE")) c->tw_mode[0] = c->tw_mode[1] = std::atoi(e);
+  if (const char* e = std::getenv("QSF_TW_MODE_F32")) c->tw_mode[1] = std::atoi(e);
   if (const char* e = std::getenv("QSF_PAIR_ONFLY")) c->pair_onfly = std::atoi(e) != 0;
   int rc = guard(nullptr, [&] {
     int count = 0;
