@@ -38,6 +38,9 @@ int qsf_ctx_create(int device, qsf_ctx** out);
 int qsf_ctx_destroy(qsf_ctx* ctx);
 const char* qsf_last_error(const qsf_ctx* ctx);
 int qsf_ctx_synchronize(qsf_ctx* ctx);
+/* Non-blocking: *idle = 1 when all work enqueued on the context's stream has finished, else 0
+ * (no counterpart in the reference; used by schedulers that keep several contexts busy). */
+int qsf_ctx_query(qsf_ctx* ctx, int* idle);
 /* Device time (ms) between the two most recent qsf_timer_mark calls, on the context stream. */
 int qsf_timer_mark(qsf_ctx* ctx);
 int qsf_timer_elapsed_ms(qsf_ctx* ctx, double* ms);

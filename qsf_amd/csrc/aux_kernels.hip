@@ -209,7 +209,16 @@ __global__ void flux_kernel(FluxArgs a) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) a.partials[blockIdx.x * nb + b] = red[0];
+    if (threadIdx.x == 0) {
+      if (gridDim.x == 1) {
+        // one block holds the whole sum: finish here (what flux_finish_kernel computes for a
+        // single partial, bit for bit) and save that launch
+        a.out[b] = red[0] * a.scale;
+        if (a.accum) a.accum[b] += a.out[b] * a.accum_weight;
+      } else {
+        a.partials[blockIdx.x * nb + b] = red[0];
+      }
+    }
     __syncthreads();
   }
 }
@@ -240,8 +249,9 @@ void launch_flux(const FluxArgs& a, hipStream_t s) {
   const int blocks = (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
   if (a.precision == QSF_FP64) hipLaunchKernelGGL(flux_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(flux_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(flux_finish_kernel, dim3(1), dim3(256), 0, s, a.partials, (int64_t)blocks, a.nb, a.scale,
-                     a.out, a.accum, a.accum_weight);
+  if (blocks > 1)
+    hipLaunchKernelGGL(flux_finish_kernel, dim3(1), dim3(256), 0, s, a.partials, (int64_t)blocks, a.nb, a.scale,
+                       a.out, a.accum, a.accum_weight);
 }
 
 template <typename Real>

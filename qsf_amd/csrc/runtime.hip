@@ -479,13 +479,16 @@ struct Grid {
     pending_tables.clear();
   }
 
-  int run(PassParams& p, int force_variant = -1) {
+  // `single_out`: where the three block sums go when the pass is one tile (they are then the
+  // final sums and need no reduction launch)
+  int run(PassParams& p, int force_variant = -1, double* single_out = nullptr) {
     build_tables();
     p.tw_mode = ctx->tw_mode[prec];
     const int var = force_variant >= 0 ? force_variant : ctx->variant_of(prec, p.inner == 1, n[p.axis], m);
     if (p.flags & F_OBS) {
       int64_t tiles = axis_pass_tiles(prec, p, var);
       partials.ensure(tiles * 3 * sizeof(double));
+      if (tiles == 1 && single_out) p.obs.partials = single_out;
       if (!p.obs.partials) p.obs.partials = partials.as<double>();
     }
     hipEvent_t ea = nullptr, eb = nullptr;
@@ -1184,10 +1187,12 @@ struct Prop {
     }
   }
 
+  double* obs_row() { return devrec.as<double>() + (size_t)pend_row * (3 + 16); }
+
   void after_obs(Grid& g, int tiles) {
     // reduce [norm, epot] of the boundary state into this row's device record
-    double* rec = devrec.as<double>() + (size_t)pend_row * (3 + 16);
-    launch_reduce_partials(g.partials.as<double>(), tiles, 3, rec, g.redscratch.as<double>(), st());
+    double* rec = obs_row();
+    if (tiles > 1) launch_reduce_partials(g.partials.as<double>(), tiles, 3, rec, g.redscratch.as<double>(), st());
     if (pd.nborders > 0 && (pd.outputs & QSF_OUT_FLUX))
       g.flux_from_gather(couples() == QSF_REP_P ? pend_obs_C : nullptr, rec + 3, fluxint.as<double>(), dt);
     if (pend_ekin) {
@@ -1233,7 +1238,7 @@ struct Prop {
       for (int a = 0; a < 3; ++a) p.pf[a] = pf[a];
       int tiles = 0;
       if (observe) obs_into(g, p);
-      tiles = g.run(p);
+      tiles = g.run(p, -1, observe ? obs_row() : nullptr);
       if (observe) after_obs(g, tiles);
       deferred[r] = true;
       pend[r].on = false;
@@ -1250,7 +1255,7 @@ struct Prop {
         if (post) add_post(g, p);
       }
       if (observe) obs_into(g, p);
-      int tiles = g.run(p);
+      int tiles = g.run(p, -1, observe ? obs_row() : nullptr);
       if (observe) after_obs(g, tiles);
     }
     for (int a = D - 2; a >= 1; --a) {  // middle axes forward
@@ -1292,7 +1297,7 @@ struct Prop {
       if (r == 0 && post) add_post(g, p);
       if (observe) obs_into(g, p);
       if (p.flags) {
-        int tiles = g.run(p);
+        int tiles = g.run(p, -1, observe ? obs_row() : nullptr);
         if (observe) after_obs(g, tiles);
       }
       deferred[r] = false;
@@ -1827,6 +1832,20 @@ const char* qsf_last_error(const qsf_ctx* c) { return c ? c->err.c_str() : g_noc
 int qsf_ctx_synchronize(qsf_ctx* c) {
   if (!c) return QSF_ERR_ARG;
   return guard(c, [&] { QSF_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+int qsf_ctx_query(qsf_ctx* c, int* idle) {
+  if (!c || !idle) return QSF_ERR_ARG;
+  return guard(c, [&] {
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipErrorNotReady) {
+      (void)hipGetLastError();   // not an error: work is still running
+      *idle = 0;
+      return;
+    }
+    QSF_HIP(e);
+    *idle = 1;
+  });
 }
 
 int qsf_timer_mark(qsf_ctx* c) {

@@ -24,6 +24,7 @@ SIGNATURES = {
     "qsf_ctx_destroy": (_I, [_P]),
     "qsf_last_error": (C.c_char_p, [_P]),
     "qsf_ctx_synchronize": (_I, [_P]),
+    "qsf_ctx_query": (_I, [_P, C.POINTER(_I)]),
     "qsf_timer_mark": (_I, [_P]),
     "qsf_timer_elapsed_ms": (_I, [_P, _dp]),
     "qsf_set_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(_P)]),
@@ -132,6 +133,12 @@ class Context:
 
     def synchronize(self):
         self.check(load().qsf_ctx_synchronize(self.h), "synchronize")
+
+    def idle(self):
+        """True when everything enqueued on this context's stream has finished (no wait)."""
+        v = C.c_int()
+        self.check(load().qsf_ctx_query(self.h, C.byref(v)), "ctx_query")
+        return bool(v.value)
 
     def mark(self):
         self.check(load().qsf_timer_mark(self.h), "timer_mark")

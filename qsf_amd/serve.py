@@ -5,11 +5,11 @@ Requests (a grid, a propagation setting and a field) arrive over time. The serve
   * admits a request when its HBM footprint fits what is left of the device budget (288 GB by
     default), first come first served, and keeps its state resident on the device until the
     result is read back;
-  * advances every admitted solve together: each tick gives each active solve a quantum of
-    steps (sized so every solve gets about the same grid-point-steps per tick), enqueued on
-    one of several HIP streams (one library context each) so kernels of different solves
-    overlap on the device; solves join at the next tick after admission and leave as soon as
-    their last step is done (continuous batching at step boundaries);
+  * advances every admitted solve together: each solve lives on one of several HIP streams
+    (one library context each, small and large solves on separate streams) and gets a quantum
+    of steps (about the same grid-point-steps for every solve) whenever its stream has drained,
+    so kernels of different solves overlap on the device; solves join after admission and
+    leave as soon as their last step is done (continuous batching at step boundaries);
   * reports each request's time to result (arrival -> result on the host), throughput and
     p50/p99.
 
@@ -29,6 +29,7 @@ import numpy as np
 from . import qsf_types as T
 
 HBM_BYTES = 288 * 10**9
+SMALL = 1 << 16   # grid points up to which a solve counts as small (its own streams)
 
 
 @dataclasses.dataclass
@@ -233,49 +234,58 @@ class Server:
         return res
 
     def run(self, requests, log=None):
-        """Serve `requests` (arrival times relative to now); return results in completion order."""
+        """Serve `requests` (arrival times relative to now); return results in completion order.
+
+        Each stream is refilled as soon as it drains (non-blocking query), so a stream of short
+        solves does not wait for a stream of long ones; small solves (<= SMALL points) have
+        streams of their own (half of them, with two or more streams)."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        active, results = [], []
+        S = len(self.ctxs)
+        n_small = S // 2 if S >= 2 else S
+        pools = (list(range(n_small)), list(range(n_small, S)) if S >= 2 else list(range(S)))
+        active = [[] for _ in range(S)]
+        results = []
         used = 0
-        load = [0.0] * len(self.ctxs)
+        load = [0] * S
         t0 = time.perf_counter()
-        while pending or active:
+        while pending or any(active):
             now = time.perf_counter() - t0
             # admission: first come first served, by footprint against the HBM budget
             for _ in range(admissible(pending, now, used, self.budget)):
                 req = pending.pop(0)
-                k = int(np.argmin(load))          # the stream holding the fewest bytes
+                m = int(np.prod([req.desc.n[i] for i in range(req.desc.dim)]))
+                pool = pools[0] if m <= SMALL else pools[1]
+                k = min(pool, key=lambda j: load[j])   # the stream holding the fewest bytes
                 a = self._start(req, self.ctxs[k], now)
                 load[k] += a.bytes
                 used += a.bytes
-                active.append((k, a))
-            if not active:
-                if pending:
-                    time.sleep(max(0.0, min(0.05, pending[0].arrival - now)))
-                continue
-            # one tick: every active solve advances by its quantum, streams run concurrently
-            for k, a in active:
-                steps = min(a.quantum, a.remaining - 1)  # the last loop body runs in finish()
-                if steps > 0:
+                active[k].append(a)
+            progressed = False
+            for k in range(S):
+                if not active[k] or not self.ctxs[k].idle():
+                    continue
+                # stream k drained: hand back its finished solves, give the rest their quantum
+                keep = []
+                for a in active[k]:
+                    if a.remaining <= 1:
+                        results.append(self._harvest(a, time.perf_counter() - t0))
+                        load[k] -= a.bytes
+                        used -= a.bytes
+                        if log:
+                            r = results[-1]
+                            log(f"request {r.rid} ({r.kind}, {r.points} pts, {r.steps} steps) done: "
+                                f"latency {r.latency:.3f} s, yield {r.ionization_yield:.6e}")
+                    else:
+                        keep.append(a)
+                active[k] = keep
+                for a in keep:
+                    steps = min(a.quantum, a.remaining - 1)  # the last loop body runs in finish()
                     done = a.prop.run_steps(steps)
                     a.remaining = a.remaining - done if done == steps else 1
-            for c in self.ctxs:
-                c.synchronize()
-            self.ticks += 1
-            now = time.perf_counter() - t0
-            keep = []
-            for k, a in active:
-                if a.remaining <= 1:
-                    results.append(self._harvest(a, time.perf_counter() - t0))
-                    load[k] -= a.bytes
-                    used -= a.bytes
-                    if log:
-                        r = results[-1]
-                        log(f"request {r.rid} ({r.kind}, {r.points} pts, {r.steps} steps) done: "
-                            f"latency {r.latency:.3f} s, yield {r.ionization_yield:.6e}")
-                else:
-                    keep.append((k, a))
-            active = keep
+                self.ticks += 1
+                progressed = True
+            if not progressed:
+                time.sleep(0.0005 if any(active) else max(0.0, min(0.05, pending[0].arrival - now)))
         return results
 
 
