@@ -324,8 +324,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = valid ? src[base + (int64_t)(t + TT * q) * step] : T{Real(0), Real(0)};
   if (p.flags & F_SCALE) {
+    // the scale (1/m of an inverse transform: 1/768 has no exact FP32 form) is applied in FP64
+    // with one rounding; a once-rounded FP32 scale would bias the norm the same way every step
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], Real(p.scale));
+    for (int q = 0; q < E; ++q) v[q] = capply(v[q], double2{p.scale, 0.0});
   }
 
   auto run_fft = [&](auto dir_tag) {
