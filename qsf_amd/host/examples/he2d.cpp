@@ -2,7 +2,8 @@
 // (Eberly interaction, CAP, velocity gauge, MultiProductSplit order 2, ZOA fluxes) written
 // with the reference's type names, propagated on the MI355X through libqsf_hip.so.
 //
-// usage: he2d <n> <steps> [multigrid 0|1]   — prints one CSV row per computeEach:
+// usage: he2d <n> <steps> [multigrid 0|1] [save_path] — prints one CSV row per computeEach
+//        (and, with save_path, writes the final state as <save_path>_repX.psi):
 //        step,time,norm,epot,ekin,etot,ediff,flux...
 // The initial state is a seeded Gaussian wave packet so a caller can rebuild it.
 #include <cmath>
@@ -13,7 +14,7 @@
 
 using namespace QSF;
 
-template <class Grid> static int run(Grid grid, long n, long steps) {
+template <class Grid> static int run(Grid grid, long n, long steps, const char* save_path) {
   using Pot = ReducedDimInteraction<ReducedModel::Eberly>;
   using Pulse = Field<AXIS::XY, Sin2Envelope<SinPulse>>;
   using Coupling = DipoleCoupling<VelocityGauge, Pulse>;
@@ -39,6 +40,7 @@ template <class Grid> static int run(Grid grid, long n, long steps) {
           });
   } catch (Stop&) {
   }
+  if (save_path) p.wf.save(save_path);
   int w = 0;
   auto rec = p.records(&w);
   for (size_t r = 0; r < rec.size() / w; ++r) {
@@ -52,9 +54,10 @@ int main(int argc, char** argv) {
   const long n = argc > 1 ? std::atol(argv[1]) : 64;
   const long steps = argc > 2 ? std::atol(argv[2]) : 10;
   const int multigrid = argc > 3 ? std::atoi(argv[3]) : 0;
+  const char* save_path = argc > 4 ? argv[4] : nullptr;
   try {
-    if (multigrid) return run(MultiCartesianGrid<2>(0.3, n), n, steps);
-    return run(CartesianGrid<2>(0.3, n), n, steps);
+    if (multigrid) return run(MultiCartesianGrid<2>(0.3, n), n, steps, save_path);
+    return run(CartesianGrid<2>(0.3, n), n, steps, save_path);
   } catch (const qsf_error& e) {
     std::fprintf(stderr, "qsf error %d: %s\n", e.code, e.what());
     return 1;

@@ -16,6 +16,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -271,6 +272,26 @@ template <class GType, class V_Op, class C_Op> struct Spin0 {
     return v;
   }
   void postCompute() { check(qsf_set_post_compute(state_->set), "qsf_set_post_compute"); }
+
+  // save() of a state in X rep as a complex `.psi` file (wf/grid.h:1025-1071): header
+  // u8 dim, u8 rep, u8 complex, i64 downscale, i64 n[dim], f64 dx[dim], u8 bounded[dim], then the
+  // row-major complex128 values. `path` gets "_repX.psi" appended; returns the file name.
+  std::string save(const std::string& path, int r = 0) const {
+    const std::string name = path + "_repX.psi";
+    std::FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f) throw qsf_error(QSF_ERR_ARG, "cannot open " + name);
+    const unsigned char head[3] = {(unsigned char)DIM, (unsigned char)QSF_REP_X, 1};
+    const int64_t downscale = 1;
+    std::fwrite(head, 1, 3, f);
+    std::fwrite(&downscale, 8, 1, f);
+    for (int i = 0; i < DIM; ++i) { const int64_t n = grid.n; std::fwrite(&n, 8, 1, f); }
+    for (int i = 0; i < DIM; ++i) { const double dx = grid.dx; std::fwrite(&dx, 8, 1, f); }
+    for (int i = 0; i < DIM; ++i) { const unsigned char b = 1; std::fwrite(&b, 1, 1, f); }
+    const std::vector<cxd> v = psi(r);
+    std::fwrite(v.data(), sizeof(cxd), v.size(), f);
+    std::fclose(f);
+    return name;
+  }
 
   void check(int rc, const char* what) const { Device::check(state_ ? state_->ctx : nullptr, rc, what); }
 

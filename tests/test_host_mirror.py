@@ -23,7 +23,7 @@ def test_host_mirror_builds_with_gxx():
     assert os.access(EXE, os.X_OK)
 
 
-def he2d_oracle(n, steps, multigrid):
+def he2d_oracle(n, steps, multigrid, with_state=False):
     """The configuration he2d.cpp writes with reference type names, run on the oracle."""
     from oracle import oracle as O
     dx = 0.3
@@ -39,7 +39,8 @@ def he2d_oracle(n, steps, multigrid):
     for r in range(len(gs)):
         gs[r].set_psi(psi0 if r == 0 else np.zeros_like(psi0))
     prop = O.Propagator(gs, p, f)
-    return prop.run(steps)
+    rec = prop.run(steps)
+    return (rec, gs[0].psi()) if with_state else rec
 
 
 @pytest.mark.gpu
@@ -47,11 +48,18 @@ def he2d_oracle(n, steps, multigrid):
 def test_host_mirror_program_matches_oracle(multigrid):
     n, steps = 64, 10
     print(f"[case] he2d n={n} steps={steps} multigrid={multigrid}", flush=True)
-    out = subprocess.run([EXE, str(n), str(steps), str(multigrid)], check=True, capture_output=True,
+    import tempfile
+    from qsf_amd import psi_file as PF
+    save = os.path.join(tempfile.mkdtemp(), "he2d")
+    out = subprocess.run([EXE, str(n), str(steps), str(multigrid), save], check=True, capture_output=True,
                          text=True, timeout=120).stdout
     rec = np.array([[float(v) for v in line.split(",")] for line in out.strip().splitlines()])
-    ref = he2d_oracle(n, steps, multigrid)
+    ref, psi_ref = he2d_oracle(n, steps, multigrid, with_state=True)
     assert rec.shape == ref.shape
+    # the state the program saved as a .psi file (main region)
+    hd, psi = PF.read(save + "_repX.psi")
+    assert hd["dim"] == 2 and hd["ns"] == [n, n] and hd["isComplex"]
+    assert np.linalg.norm(psi.ravel() - psi_ref) / np.linalg.norm(psi_ref) < 1e-10
     np.testing.assert_allclose(rec[:, :2], ref[:, :2], rtol=0, atol=1e-12)
     np.testing.assert_allclose(rec[:, 2:4], ref[:, 2:4], rtol=1e-11, atol=1e-13)
     np.testing.assert_allclose(rec[:, 7:], ref[:, 7:], rtol=1e-8, atol=1e-12 * np.abs(ref[:, 7:]).max())
