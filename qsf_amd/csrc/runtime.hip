@@ -1021,6 +1021,11 @@ struct PendingX {
   XFactor f{};
 };
 
+// The propagator driving each set's grids (the last one created on it); entry points that
+// read or change a grid settle it first.
+struct Prop;
+static std::map<const Set*, Prop*> g_active;
+
 struct Prop {
   Set* set;
   qsf_prop_desc pd;
@@ -1047,6 +1052,7 @@ struct Prop {
   bool pend_ekin = false;
 
   Prop(Set* s, const qsf_prop_desc& p, const qsf_field_desc* f, int nf) : set(s), pd(p), coup(p.gauge, f, nf) {
+    g_active[s] = this;
     QSF_REQUIRE(p.order >= 1 && p.order <= 8, QSF_ERR_ARG, "order must be 1..8");
     QSF_REQUIRE(p.nborders >= 0 && p.nborders <= 16, QSF_ERR_ARG, "0..16 borders");
     dt = p.dt;
@@ -1392,6 +1398,22 @@ struct Prop {
   void flush_obs_only() { flush(); }
 
   bool fused() const { return exec == 0; }
+  ~Prop() {
+    for (auto it = g_active.begin(); it != g_active.end(); ++it)
+      if (it->second == this) {
+        g_active.erase(it);
+        break;
+      }
+  }
+  Prop(const Prop&) = delete;
+  Prop& operator=(const Prop&) = delete;
+
+  // A caller that reads or changes a grid between steps (a worker at WHEN::DURING,
+  // propagator.h:446-452) must see the step-boundary state: apply what the fused pipeline
+  // left pending (inverse transform, trailing X half, observation, multigrid transfer).
+  void settle() {
+    if (fused()) flush();
+  }
 
   void compute_each() {
     if (pd.outputs == 0) return;
@@ -1694,7 +1716,7 @@ struct Batch {
 using namespace qsf;
 
 struct qsf_set { Set s; qsf_set(qsf_ctx* c, const qsf_grid_desc& d) : s(c, d) {} };
-struct qsf_grid { Grid* g; };
+struct qsf_grid { Grid* g; Set* set = nullptr; };
 struct qsf_prop { Prop p; qsf_prop(Set* s, const qsf_prop_desc& d, const qsf_field_desc* f, int nf) : p(s, d, f, nf) {} };
 
 struct qsf_batch { Batch b; qsf_batch(qsf_ctx* c, const qsf_grid_desc* d, const qsf_field_desc* f, int n,
@@ -1716,14 +1738,22 @@ template <typename F> int guard(qsf_ctx* ctx, F&& f) {
   }
 }
 std::map<const qsf_grid*, std::unique_ptr<qsf_grid>> g_handles;
-qsf_grid* handle_of(Grid* g) {
+qsf_grid* handle_of(Grid* g, Set* s) {
   for (auto& kv : g_handles)
     if (kv.second->g == g) return kv.second.get();
   auto h = std::make_unique<qsf_grid>();
   h->g = g;
+  h->set = s;
   qsf_grid* p = h.get();
   g_handles[p] = std::move(h);
   return p;
+}
+void settle(const Set* s) {
+  auto it = g_active.find(s);
+  if (it != g_active.end()) it->second->settle();
+}
+void settle(qsf_grid* g) {
+  if (g->set) settle(g->set);
 }
 }  // namespace
 
@@ -1880,6 +1910,7 @@ int qsf_set_create(qsf_ctx* c, const qsf_grid_desc* d, qsf_set** out) {
 int qsf_set_destroy(qsf_set* s) {
   if (!s) return QSF_ERR_ARG;
   (void)hipStreamSynchronize(s->s.ctx->stream);
+  g_active.erase(&s->s);
   for (auto& g : s->s.g) {
     for (auto it = g_handles.begin(); it != g_handles.end();)
       if (it->second->g == g.get()) it = g_handles.erase(it);
@@ -1897,13 +1928,16 @@ int qsf_set_count(const qsf_set* s, int* n) {
 
 int qsf_set_grid(qsf_set* s, int r, qsf_grid** out) {
   if (!s || !out || r < 0 || r >= (int)s->s.g.size()) return QSF_ERR_ARG;
-  *out = handle_of(s->s.g[r].get());
+  *out = handle_of(s->s.g[r].get(), &s->s);
   return QSF_OK;
 }
 
 int qsf_set_post_compute(qsf_set* s) {
   if (!s) return QSF_ERR_ARG;
-  return guard(s->s.ctx, [&] { s->s.post_compute(); });
+  return guard(s->s.ctx, [&] {
+    settle(&s->s);
+    s->s.post_compute();
+  });
 }
 
 int qsf_grid_size(const qsf_grid* g, int64_t* m) {
@@ -1918,19 +1952,29 @@ int qsf_grid_rep(const qsf_grid* g, int* r) {
 }
 int qsf_grid_upload(qsf_grid* g, const double* h, int64_t count, int rep) {
   if (!g || !h) return QSF_ERR_ARG;
-  return guard(g->g->ctx, [&] { g->g->upload(h, count, rep); });
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    g->g->upload(h, count, rep);
+  });
 }
 int qsf_grid_download(qsf_grid* g, double* h, int64_t count) {
   if (!g || !h) return QSF_ERR_ARG;
-  return guard(g->g->ctx, [&] { g->g->download(h, count); });
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    g->g->download(h, count);
+  });
 }
 int qsf_fourier(qsf_grid* g, int to) {
   if (!g) return QSF_ERR_ARG;
-  return guard(g->g->ctx, [&] { g->g->fourier(to); });
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    g->g->fourier(to);
+  });
 }
 int qsf_evolve(qsf_grid* g, int mode, int rep, double delta, const double* C, int couples_in) {
   if (!g) return QSF_ERR_ARG;
   return guard(g->g->ctx, [&] {
+    settle(g);
     QSF_REQUIRE(rep == g->g->rep, QSF_ERR_STATE, "evolve: grid is in the other representation");
     QSF_REQUIRE(mode == QSF_MODE_IM || mode == QSF_MODE_RE, QSF_ERR_ARG, "bad mode");
     g->g->evolve(mode, rep, delta, C, couples_in);
@@ -1939,6 +1983,7 @@ int qsf_evolve(qsf_grid* g, int mode, int rep, double delta, const double* C, in
 int qsf_average(qsf_grid* g, int rep, int op, double* out) {
   if (!g || !out) return QSF_ERR_ARG;
   return guard(g->g->ctx, [&] {
+    settle(g);
     QSF_REQUIRE(rep == g->g->rep, QSF_ERR_STATE, "average: grid is in the other representation");
     QSF_REQUIRE(op >= 0 && op <= 2, QSF_ERR_ARG, "op must be 0..2");
     *out = g->g->average(rep, op);
@@ -1946,15 +1991,24 @@ int qsf_average(qsf_grid* g, int rep, int op, double* out) {
 }
 int qsf_normalize(qsf_grid* g) {
   if (!g) return QSF_ERR_ARG;
-  return guard(g->g->ctx, [&] { g->g->normalize(); });
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    g->g->normalize();
+  });
 }
 int qsf_flux(qsf_grid* g, const double* vgC, const qsf_border_desc* b, int nb, double* out) {
   if (!g || !b || !out) return QSF_ERR_ARG;
-  return guard(g->g->ctx, [&] { g->g->flux(vgC, b, nb, out); });
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    g->g->flux(vgC, b, nb, out);
+  });
 }
 int qsf_mask_region(qsf_grid* g) {
   if (!g) return QSF_ERR_ARG;
-  return guard(g->g->ctx, [&] { g->g->mask_region(); });
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    g->g->mask_region();
+  });
 }
 
 int qsf_prop_create(qsf_set* s, const qsf_prop_desc* d, const qsf_field_desc* f, int nf, qsf_prop** out) {

@@ -275,6 +275,67 @@ def test_propagation_multigrid_2d(ctx):
         assert np.allclose(rd[:, 2], ro[:, 2], rtol=1e-11)
 
 
+@pytest.mark.parametrize("multigrid", [0, 1])
+def test_state_between_steps(ctx, multigrid):
+    # A worker at WHEN::DURING (propagator.h:446-452) reads and may change the state between
+    # steps. The grid entry points first apply what the fused pipeline left pending (inverse
+    # transform, trailing X half, multigrid transfer), so the state read is the oracle's after the
+    # same steps, a state written is the one the next step starts from, and the records of the
+    # interrupted run equal those of an uninterrupted one.
+    from qsf_amd import hip
+    n, dx, steps = 64, 0.3, 10
+    d = T.grid_desc(2, n, dx, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                    absorber=T.ABS_CAP, nCAP=8, multigrid=multigrid)
+    psi0 = crand(n * n) * 0.01
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    pdesc = T.prop_desc(T.MODE_RE, 2, 0.02, gauge=T.GAUGE_VELOCITY,
+                        outputs=T.OUT_NORM | T.OUT_EPOT | T.OUT_FLUX, borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+    print(f"[case] state between steps multigrid={multigrid}", flush=True)
+
+    def loaded(gs):
+        for r in range(len(gs)):
+            gs[r].set_psi(psi0 if r == 0 else np.zeros_like(psi0))
+        return gs
+
+    dev, ref = loaded(hip.GridSet(ctx, d)), loaded(O.GridSet(d))
+    pd, po = hip.Propagator(dev, pdesc, f), O.Propagator(ref, pdesc, f)
+    pd.start()
+    po.start()
+    for k in range(steps):
+        pd.step()
+        po.step()
+        if k in (3, 6):
+            for r in range(len(ref)):
+                assert rel(dev[r].psi(), ref[r].psi()) < FP64_TOL, f"step {k} region {r}"
+    rd, ro = pd.records(), po.records()
+    np.testing.assert_allclose(rd[:, 2:4], ro[:, 2:4], rtol=1e-11)
+    assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
+
+    plain = loaded(hip.GridSet(ctx, d))
+    rp = hip.Propagator(plain, pdesc, f).run(steps)
+    assert rd.shape == rp.shape
+    np.testing.assert_allclose(rd, rp, rtol=1e-12, atol=1e-14 * np.nanmax(np.abs(rp)))
+    for r in range(len(ref)):
+        assert rel(dev[r].psi(), plain[r].psi()) < 1e-12
+
+    # a state written between steps is where the next step starts
+    dev, ref = loaded(hip.GridSet(ctx, d)), loaded(O.GridSet(d))
+    pd, po = hip.Propagator(dev, pdesc, f), O.Propagator(ref, pdesc, f)
+    pd.start()
+    po.start()
+    for k in range(6):
+        pd.step()
+        po.step()
+        if k == 2:
+            half = 0.5 * ref[0].psi()
+            dev[0].set_psi(half)
+            ref[0].set_psi(half)
+    rd, ro = pd.records(), po.records()
+    for r in range(len(ref)):
+        assert rel(dev[r].psi(), ref[r].psi()) < FP64_TOL, f"region {r}"
+    np.testing.assert_allclose(rd[:, 2:4], ro[:, 2:4], rtol=1e-11)
+
+
 def test_imaginary_time_ground_state(ctx):
     n, dx = 256, 0.2
     d = T.grid_desc(1, n, dx, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1)
