@@ -117,6 +117,29 @@ def run_c4(steps, warmup):
     wall = time.perf_counter() - t0
     pts = sum(int(d.n[0]) * int(d.n[1]) for d in descs32)
     batched = pts * steps / wall
+    # every solve checked against the same solve advanced in FP64 for the same steps
+    progress("checking each solve against an FP64 batch")
+    descs64, _ = c4_solves(T.FP64)
+    b64 = hip.Batch(ctx, descs64, fields, pd)
+    for s, d in enumerate(descs64):
+        b64.set_psi(s, ground_like_state(d.n[0]))
+    b64.start()
+    b64.run(warmup + steps, use_graph=True)
+    ctx.synchronize()
+    # a batch state downloads only after its whole pulse; mid-run, every observed record row
+    # (norm, fluxes) and the FP64 flux time integrals are compared at the same step
+    worst = dict(norm_rel_err=0.0, flux_integral_max_rel_err=0.0)
+    for s in range(len(descs32)):
+        r32, r64 = b.records(s), b64.records(s)
+        k = min(len(r32), len(r64))
+        worst["norm_rel_err"] = max(worst["norm_rel_err"],
+                                    float(np.max(np.abs(r32[:k, T.REC_NORM] / r64[:k, T.REC_NORM] - 1))))
+        f32, f64 = b.flux_integral(s), b64.flux_integral(s)
+        scale = np.abs(f64).max()
+        if scale > 0:
+            worst["flux_integral_max_rel_err"] = max(worst["flux_integral_max_rel_err"],
+                                                     float(np.abs(f32 - f64).max() / scale))
+    b64.close()
     b.close()
     # the plain FP64 step: one propagator per solve, per-size step time
     per_size = {}
@@ -138,7 +161,8 @@ def run_c4(steps, warmup):
     ctx.close()
     return dict(value=batched, ms_per_step=wall / steps * 1e3, pts=pts, plain=plain,
                 plain_ms_per_step=plain_s * 1e3, per_size_ms={k: round(v * 1e3, 4) for k, v in per_size.items()},
-                sides={int(n): int(sum(1 for d in descs32 if d.n[0] == n)) for n in C4_SIDES})
+                sides={int(n): int(sum(1 for d in descs32 if d.n[0] == n)) for n in C4_SIDES},
+                vs_fp64=dict(worst, steps=warmup + steps, solves=len(descs32)))
 
 
 def run_c5(count, streams=4):
@@ -343,7 +367,8 @@ def main():
                           "grid_points_per_step": r["pts"], "solves_per_side": r["sides"]},
                "plain_fp64": {"value": round(r["plain"], 1), "ms_per_step": round(r["plain_ms_per_step"], 4),
                               "per_size_ms_per_step": r["per_size_ms"]},
-               "speedup_vs_plain_fp64": round(r["value"] / r["plain"], 3)}
+               "speedup_vs_plain_fp64": round(r["value"] / r["plain"], 3),
+               "vs_fp64_same_steps": r["vs_fp64"]}
         print(json.dumps(out))
         return
     if args.workload == "c2":
