@@ -101,6 +101,11 @@ int qsf_prop_create(qsf_set* set, const qsf_prop_desc* desc, const qsf_field_des
 int qsf_prop_destroy(qsf_prop* p);
 int qsf_prop_max_steps(const qsf_prop* p, int64_t* out);      /* propagator.h:153 */
 int qsf_prop_start(qsf_prop* p);                              /* computeEach<AT_START> :439-441 */
+/* Resume in place of qsf_prop_start: makeFakeStep until step == at_step (propagator.h:347-366,
+ * 425-436): each chain's precalc side effects and timer increments, no evolution, no chain
+ * timer restore (so an order-2 MultiProductSplit clock advances by both chains' sums per fake
+ * step, as in the reference). No AT_START row. The caller then uploads the saved state. */
+int qsf_prop_fast_forward(qsf_prop* p, int64_t at_step);
 int qsf_prop_should_continue(qsf_prop* p, int* out);          /* shouldContinue :304-309 */
 int qsf_prop_step(qsf_prop* p);                               /* one loop body :445-449 */
 int qsf_prop_run_steps(qsf_prop* p, int64_t k, int64_t* done); /* up to k loop bodies, no host sync */

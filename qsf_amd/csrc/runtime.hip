@@ -1102,6 +1102,18 @@ struct Prop {
     return ops;
   }
 
+  // makeFakeStep (propagator.h:347-366): every chain's precalc side effects and timer
+  // increments with no evolution and, unlike makeStep, no chainBackup/Restore of the timer.
+  void fake_step() {
+    for (const Chain& ch : chains)
+      for (size_t SI = 0; SI < ch.reps.size(); ++SI)
+        if (couples() == ch.reps[SI]) {
+          coup.precalc(timer);
+          timer += dt * (ch.mults[SI] * 1.0);
+        }
+    ++step;
+  }
+
   // ---- unfused execution: one standalone call per reference member ----
   void exec_simple(const std::vector<SubOp>& ops) {
     for (size_t SI = 0; SI < ops.size(); ++SI) {
@@ -2030,6 +2042,14 @@ int qsf_prop_max_steps(const qsf_prop* p, int64_t* o) {
 int qsf_prop_start(qsf_prop* p) {
   if (!p) return QSF_ERR_ARG;
   return guard(p->p.set->ctx, [&] { p->p.start(); });
+}
+int qsf_prop_fast_forward(qsf_prop* p, int64_t at_step) {
+  if (!p || at_step < 0) return QSF_ERR_ARG;
+  return guard(p->p.set->ctx, [&] {
+    QSF_REQUIRE(p->p.step == 0 && p->p.rows.empty(), QSF_ERR_STATE,
+                "fast_forward replaces start: call it on a fresh propagator");
+    while (p->p.step < at_step) p->p.fake_step();
+  });
 }
 int qsf_prop_should_continue(qsf_prop* p, int* o) {
   if (!p || !o) return QSF_ERR_ARG;

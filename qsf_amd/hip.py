@@ -25,6 +25,7 @@ SIGNATURES = {
     "qsf_last_error": (C.c_char_p, [_P]),
     "qsf_ctx_synchronize": (_I, [_P]),
     "qsf_ctx_query": (_I, [_P, C.POINTER(_I)]),
+    "qsf_prop_fast_forward": (_I, [_P, _I64]),
     "qsf_timer_mark": (_I, [_P]),
     "qsf_timer_elapsed_ms": (_I, [_P, _dp]),
     "qsf_set_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(_P)]),
@@ -352,7 +353,10 @@ class Propagator:
             self.ctx.check(load().qsf_prop_set_exec(h, exec_mode), "prop_set_exec")
 
     def close(self):
-        if getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None):
+        # the C propagator reaches its set's stream: once the set is gone (a garbage cycle can
+        # finalize the set first, after its weak list of propagators was cleared) it is left
+        if (getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None)
+                and getattr(self.gset, "h", None)):
             load().qsf_prop_destroy(self.h)
         self.h = None
 
@@ -388,6 +392,10 @@ class Propagator:
 
     def start(self):
         self.ctx.check(load().qsf_prop_start(self.h), "prop_start")
+
+    def fast_forward(self, at_step):
+        """Resume in place of start(): makeFakeStep until step == at_step (propagator.h:425-436)."""
+        self.ctx.check(load().qsf_prop_fast_forward(self.h, int(at_step)), "prop_fast_forward")
 
     def should_continue(self):
         v = C.c_int()

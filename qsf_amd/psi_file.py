@@ -12,6 +12,7 @@ forward (wf/grid.h:1073-1107, phaseShiftTrick) and scaled by dV / (2 pi)^(dim/2)
 square). File names get "_scale<k>" when downscaled and "_repX" / "_repP", then ".psi"
 (wf/grid.h:1040-1050).
 """
+import os
 import struct
 
 import numpy as np
@@ -120,3 +121,55 @@ def load_grid(gset, name, region=0):
         raise ValueError(f"grid shape {hd['ns']} does not match the file")
     gset[region].set_psi(np.asarray(data).ravel())
     return hd
+
+
+# ---- evolution backups (wf/grid.h:1115-1132, multigrid.h:370; resume: propagator.h:425-436) ----
+BACKUP_INFO = "backup.info"
+
+
+def backup_names(gset, directory="."):
+    """backup.psi for a single grid; backup.psi<r> per region of a multigrid set."""
+    base = os.path.join(str(directory), "backup")
+    if len(gset.grids) == 1:
+        return [base + EXT]
+    return [base + EXT + str(r) for r in range(len(gset.grids))]
+
+
+def backup(gset, step, directory="."):
+    """_backup(step): backup.info holds the step as one i64; each region's X state goes to an
+    evolution-backup file (same header, no rep suffix in the name). Taken between steps, the
+    grid entry points hand back the step-boundary state."""
+    with open(os.path.join(str(directory), BACKUP_INFO), "wb") as f:
+        f.write(struct.pack("<q", int(step)))
+    d = gset.desc
+    n = [int(d.n[i]) for i in range(d.dim)]
+    dx = [float(d.dx[i]) for i in range(d.dim)]
+    names = backup_names(gset, directory)
+    for r, name in enumerate(names):
+        # the region's bounded flags (mcomm.bounded, wf/grid.h:1035); multigrid regions run
+        # NO, X, Y, XY (mpi_logic.h:256-259), so region r frees the axes of mask r
+        free = r if len(names) > 1 else 0
+        bounded = [not (free >> i) & 1 for i in range(d.dim)]
+        with open(name, "wb") as f:
+            f.write(header_bytes(d.dim, T.REP_X, True, 1, n, dx, bounded))
+            f.write(np.ascontiguousarray(gset[r].psi(), dtype=np.complex128).tobytes())
+    return names
+
+
+def restore(gset, prop, directory="."):
+    """run(restore=true): read backup.info, fast-forward the propagator to that step
+    (makeFakeStep: coupling and clock only), then load every region's backup. Returns the step."""
+    with open(os.path.join(str(directory), BACKUP_INFO), "rb") as f:
+        (step,) = struct.unpack("<q", f.read(8))
+    prop.fast_forward(step)
+    for r, name in enumerate(backup_names(gset, directory)):
+        _, data = read(name)
+        gset[r].set_psi(np.asarray(data).ravel())
+    return step
+
+
+def remove_backups(gset, directory="."):
+    """remove_backups + the backup.info removal at the end of run (propagator.h:456-461)."""
+    for name in backup_names(gset, directory) + [os.path.join(str(directory), BACKUP_INFO)]:
+        if os.path.exists(name):
+            os.remove(name)

@@ -82,3 +82,92 @@ def test_device_grid_save_load_and_centred_p(tmp_path):
     gs2.close()
     gs.close()
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("multigrid", [0, 1])
+def test_backup_and_resume_equal_uninterrupted_run(tmp_path, multigrid):
+    # _backup at a step boundary, then run(restore=true) on fresh objects: makeFakeStep to the
+    # saved step, load every region, continue. Strang keeps one chain, so the resumed run is the
+    # uninterrupted one (propagator.h:425-449).
+    import struct
+    from qsf_amd import hip
+    n, dx, k1, k2 = 64, 0.3, 6, 12
+    d = T.grid_desc(2, n, dx, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                    absorber=T.ABS_CAP, nCAP=8, multigrid=multigrid)
+    rng = np.random.default_rng(11)
+    psi0 = (rng.normal(size=n * n) + 1j * rng.normal(size=n * n)) * 0.01
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.02, gauge=T.GAUGE_VELOCITY,
+                        outputs=T.OUT_NORM | T.OUT_FLUX, borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+    print(f"[case] backup/resume multigrid={multigrid}", flush=True)
+    ctx = hip.Context(0)
+
+    def loaded():
+        gs = hip.GridSet(ctx, d)
+        for r in range(len(gs.grids)):
+            gs[r].set_psi(psi0 if r == 0 else np.zeros_like(psi0))
+        return gs
+
+    ref = loaded()
+    pr = hip.Propagator(ref, pdesc, f)
+    pr.start()
+    for _ in range(k2):
+        pr.step()
+    rows_ref = pr.records()
+
+    first = loaded()
+    p1 = hip.Propagator(first, pdesc, f)
+    p1.start()
+    for _ in range(k1):
+        p1.step()
+    names = PF.backup(first, p1.step_count, tmp_path)
+    assert struct.unpack("<q", open(tmp_path / "backup.info", "rb").read())[0] == k1
+    assert [os.path.basename(x) for x in names] == (["backup.psi"] if not multigrid else
+                                                    [f"backup.psi{r}" for r in range(4)])
+    hd, _ = PF.read(names[-1])
+    assert hd["bounded"] == ([True, True] if not multigrid else [False, False])
+    p1.close()
+    first.close()
+
+    second = hip.GridSet(ctx, d)
+    p2 = hip.Propagator(second, pdesc, f)
+    assert PF.restore(second, p2, tmp_path) == k1
+    assert p2.step_count == k1 and abs(p2.time - k1 * 0.02) < 1e-15
+    for _ in range(k2 - k1):
+        p2.step()
+    rows = p2.records()
+    assert rows.shape[0] == k2 - k1          # no AT_START row on resume
+    np.testing.assert_allclose(rows, rows_ref[k1 + 1:], rtol=1e-12,
+                               atol=1e-14 * np.nanmax(np.abs(rows_ref)))
+    for r in range(len(ref.grids)):
+        a, b = second[r].psi(), ref[r].psi()
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+    PF.remove_backups(second, tmp_path)
+    assert not os.listdir(tmp_path)
+    for o in (p2, pr):
+        o.close()
+    second.close()
+    ref.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_fast_forward_clock_follows_fake_steps():
+    # makeFakeStep advances the clock by every chain's P-substep sum with no chain restore:
+    # Strang dt per step, MultiProductSplit order 2 (chains 1 + 1) 2 dt per step
+    from qsf_amd import hip
+    d = T.grid_desc(1, 64, 0.2, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1)
+    f = [T.field_desc(T.AXIS_X, 0.05, 0.057, 2)]
+    ctx = hip.Context(0)
+    for order, per_step in ((1, 0.05), (2, 0.1)):
+        gs = hip.GridSet(ctx, d)
+        p = hip.Propagator(gs, T.prop_desc(T.MODE_RE, order, 0.05, gauge=T.GAUGE_VELOCITY, outputs=T.OUT_NORM), f)
+        p.fast_forward(5)
+        assert p.step_count == 5
+        assert abs(p.time - 5 * per_step) < 1e-14
+        with pytest.raises(Exception):
+            p.fast_forward(7)                 # only on a fresh propagator
+        p.close()
+        gs.close()
+    ctx.close()
