@@ -93,6 +93,10 @@ int qsf_flux(qsf_grid* g, const double* vg_coupling3, const qsf_border_desc* bor
              double* out);
 /* maskRegion() grid.h:582-597 (bit-exact: host-built FP64 mask table). */
 int qsf_mask_region(qsf_grid* g);
+/* operation<M, X, AntiSymmetrize<particles>> (wf/grid.h:611-723), in place on the X state:
+ * particles 2 swaps the two coordinates of a 2D grid (the last two of a 3D grid), particles 3
+ * sums the six permutations of a cubic 3D grid; nothing for dim < particles. */
+int qsf_antisymmetrize(qsf_grid* g, int particles);
 
 /* ---- propagator: SplitPropagator<MODE, MultiProductSplit<Split3Base<X,P,X>,order>, ...>
  *      (propagator.h:51-499) over a grid set, with the step executed as fused axis passes ---- */

@@ -1,0 +1,77 @@
+"""Oracle for operation<AntiSymmetrize<...>> (wf/grid.h:611-723), one rank, X representation.
+Test infrastructure only: the tests compare the device entry point qsf_antisymmetrize with it.
+
+`antisymmetrize` is the vectorised restatement; `antisymmetrize_loops` transcribes the
+reference's loops node by node (pure Python, tiny grids) and pins it.
+"""
+import numpy as np
+
+
+def antisymmetrize(psi, n, dim, particles):
+    """psi: complex array of n^dim nodes, row-major. Returns the new array (input untouched)."""
+    a = np.array(psi, dtype=np.complex128).reshape([n] * dim)
+    if dim < particles:
+        return a.ravel()
+    if particles == 2 and dim == 2:                     # :700-716 (psi_total = original)
+        return (a - a.T).ravel()
+    if particles == 2 and dim == 3:                     # :686-699, last two axes
+        out = a - a.transpose(0, 2, 1)
+        j, k = np.indices((n, n))
+        lower = k < j
+        # (i,k,j) = -(i,j,k) for k < j: copy the negated lower triangle over the upper one
+        up = out.transpose(0, 2, 1).copy()
+        out = np.where(lower[None], out, -up)
+        out[:, np.arange(n), np.arange(n)] = 0
+        return out.ravel()
+    # particles == 3, dim == 3 (:646-677)
+    t = a.transpose(1, 2, 0) + a.transpose(2, 0, 1)     # (j,k,i) + (k,i,j) seen from (i,j,k)
+    t = t - a.transpose(1, 0, 2)
+    t = t - a.transpose(0, 2, 1)
+    t = t - a.transpose(2, 1, 0)
+    v = a + t
+    i, j, k = np.indices((n, n, n))
+    out = np.zeros_like(a)
+    s = (i > j) & (j > k)
+    I, J, K = i[s], j[s], k[s]
+    val = v[s]
+    for (p, q, r), sign in (((I, J, K), 1), ((J, K, I), 1), ((K, I, J), 1),
+                            ((J, I, K), -1), ((I, K, J), -1), ((K, J, I), -1)):
+        out[p, q, r] = sign * val
+    return out.ravel()
+
+
+def antisymmetrize_loops(psi, n, dim, particles):
+    """The reference's loops, node by node (tiny grids only)."""
+    p = [complex(x) for x in np.asarray(psi).ravel()]
+    off = (lambda i, j: i * n + j) if dim == 2 else (lambda i, j, k: (i * n + j) * n + k)
+    if dim < particles:
+        return np.array(p)
+    if particles == 2 and dim == 2:
+        total = list(p)
+        for i in range(n):
+            for j in range(n):
+                p[off(i, j)] -= total[off(j, i)]
+    elif particles == 2:
+        for i in range(n):
+            for j in range(n):
+                for k in range(j):
+                    p[off(i, j, k)] -= p[off(i, k, j)]
+                    p[off(i, k, j)] = -p[off(i, j, k)]
+                p[off(i, j, j)] = 0j
+    else:
+        for i in range(n):
+            for j in range(i):
+                for k in range(j):
+                    r, r2, r3 = off(i, j, k), off(j, k, i), off(k, i, j)
+                    s1, s2, s3 = off(j, i, k), off(i, k, j), off(k, j, i)
+                    p[r] += p[r2] + p[r3] - p[s1] - p[s2] - p[s3]
+                    p[r2] = p[r]
+                    p[r3] = p[r]
+                    p[s1] = -p[r]
+                    p[s2] = -p[r]
+                    p[s3] = -p[r]
+            for j in range(n):
+                p[off(i, j, j)] = 0j
+                p[off(j, i, j)] = 0j
+                p[off(j, j, i)] = 0j
+    return np.array(p)

@@ -338,6 +338,90 @@ __global__ void add_kernel(T* dst, const T* src, int64_t m) {
   }
 }
 
+// operation<AntiSymmetrize<...>> (wf/grid.h:611-723), in place, X representation. Every node
+// has exactly one writer: the thread of its sorted pair / triple, or, for a node with repeated
+// indices, its own thread.
+// AntiSymmetrize<2_D> on a 2D grid (:700-716, psi_total is the unmodified state):
+// psi(i,j) - psi(j,i).
+template <typename T>
+__global__ void antisym2_kernel(T* psi, int64_t n) {
+  const int64_t m = n * n;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q / n, j = q % n;
+    if (j > i) continue;
+    const int64_t b = j * n + i;
+    const T x = psi[q], y = psi[b];
+    psi[q] = T{x.x - y.x, x.y - y.y};
+    if (b != q) psi[b] = T{y.x - x.x, y.y - x.y};
+  }
+}
+// AntiSymmetrize<2_D> on a 3D grid, over the last two axes (:686-699): (i,j,k) -= (i,k,j) for
+// k < j, (i,k,j) = -(i,j,k), (i,j,j) = 0.
+template <typename T>
+__global__ void antisym2_partial_kernel(T* psi, int64_t n0, int64_t n) {
+  const int64_t nn = n * n, m = n0 * nn;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q / nn, r = q % nn, j = r / n, k = r % n;
+    if (k > j) continue;
+    if (k == j) {
+      psi[q] = T{0, 0};
+      continue;
+    }
+    const int64_t b = (i * n + k) * n + j;
+    const T x = psi[q], y = psi[b];
+    const T v{x.x - y.x, x.y - y.y};
+    psi[q] = v;
+    psi[b] = T{-v.x, -v.y};
+  }
+}
+// AntiSymmetrize<3_D>, one rank (:646-677): for i > j > k,
+// (i,j,k) += (j,k,i) + (k,i,j) - (j,i,k) - (i,k,j) - (k,j,i) in that order, the cyclic
+// permutations get the value, the transpositions its negative; nodes with two equal indices 0.
+template <typename T>
+__global__ void antisym3_kernel(T* psi, int64_t n) {
+  const int64_t nn = n * n, m = n * nn;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = q / nn, r = q % nn, j = r / n, k = r % n;
+    if (i == j || j == k || i == k) {
+      psi[q] = T{0, 0};
+      continue;
+    }
+    if (!(i > j && j > k)) continue;
+    const int64_t r2 = (j * n + k) * n + i, r3 = (k * n + i) * n + j;
+    const int64_t s1 = (j * n + i) * n + k, s2 = (i * n + k) * n + j, s3 = (k * n + j) * n + i;
+    const T a = psi[q], b = psi[r2], c = psi[r3], d = psi[s1], e = psi[s2], f = psi[s3];
+    T t{b.x + c.x, b.y + c.y};
+    t = T{t.x - d.x, t.y - d.y};
+    t = T{t.x - e.x, t.y - e.y};
+    t = T{t.x - f.x, t.y - f.y};
+    const T v{a.x + t.x, a.y + t.y}, w{-v.x, -v.y};
+    psi[q] = v;
+    psi[r2] = v;
+    psi[r3] = v;
+    psi[s1] = w;
+    psi[s2] = w;
+    psi[s3] = w;
+  }
+}
+
+// kind 2: AntiSymmetrize<2_D> (2D whole grid, 3D last two axes); kind 3: AntiSymmetrize<3_D>.
+void launch_antisym(int precision, void* psi, int dim, const int64_t* n, int kind, hipStream_t s) {
+  int64_t m = 1;
+  for (int a = 0; a < dim; ++a) m *= n[a];
+  const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
+  const bool f64 = precision == QSF_FP64;
+  if (kind == 2 && dim == 2) {
+    if (f64) hipLaunchKernelGGL(antisym2_kernel<double2>, dim3(blocks), dim3(256), 0, s, (double2*)psi, n[0]);
+    else hipLaunchKernelGGL(antisym2_kernel<float2>, dim3(blocks), dim3(256), 0, s, (float2*)psi, n[0]);
+  } else if (kind == 2 && dim == 3) {
+    if (f64) hipLaunchKernelGGL(antisym2_partial_kernel<double2>, dim3(blocks), dim3(256), 0, s, (double2*)psi, n[0], n[1]);
+    else hipLaunchKernelGGL(antisym2_partial_kernel<float2>, dim3(blocks), dim3(256), 0, s, (float2*)psi, n[0], n[1]);
+  } else if (kind == 3 && dim == 3) {
+    if (f64) hipLaunchKernelGGL(antisym3_kernel<double2>, dim3(blocks), dim3(256), 0, s, (double2*)psi, n[0]);
+    else hipLaunchKernelGGL(antisym3_kernel<float2>, dim3(blocks), dim3(256), 0, s, (float2*)psi, n[0]);
+  }
+}
+
 void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s) {
   const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
   if (precision == QSF_FP64)

@@ -2015,6 +2015,25 @@ int qsf_flux(qsf_grid* g, const double* vgC, const qsf_border_desc* b, int nb, d
     g->g->flux(vgC, b, nb, out);
   });
 }
+extern "C++" {
+namespace qsf {
+void launch_antisym(int precision, void* psi, int dim, const int64_t* n, int kind, hipStream_t s);
+}
+}
+int qsf_antisymmetrize(qsf_grid* g, int particles) {
+  if (!g || (particles != 2 && particles != 3)) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    Grid& G = *g->g;
+    if (G.D < particles) return;  // the reference does nothing for DIM < particles (grid.h:613, 684)
+    QSF_REQUIRE(G.rep == QSF_REP_X, QSF_ERR_STATE, "antisymmetrize acts on the X representation");
+    if (particles == 2 && G.D == 2) QSF_REQUIRE(G.n[0] == G.n[1], QSF_ERR_ARG, "needs n0 == n1");
+    if (G.D == 3) QSF_REQUIRE(G.n[1] == G.n[2] && (particles == 2 || G.n[0] == G.n[1]), QSF_ERR_ARG,
+                              "needs equal axis lengths");
+    launch_antisym(G.prec, G.psi.p, G.D, G.n, particles, G.st());
+    QSF_HIP(hipGetLastError());
+  });
+}
 int qsf_mask_region(qsf_grid* g) {
   if (!g) return QSF_ERR_ARG;
   return guard(g->g->ctx, [&] {

@@ -26,6 +26,7 @@ SIGNATURES = {
     "qsf_ctx_synchronize": (_I, [_P]),
     "qsf_ctx_query": (_I, [_P, C.POINTER(_I)]),
     "qsf_prop_fast_forward": (_I, [_P, _I64]),
+    "qsf_antisymmetrize": (_I, [_P, _I]),
     "qsf_timer_mark": (_I, [_P]),
     "qsf_timer_elapsed_ms": (_I, [_P, _dp]),
     "qsf_set_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(_P)]),
@@ -256,6 +257,10 @@ class Grid:
 
     def fourier(self, rep):
         self.ctx.check(load().qsf_fourier(self.h, rep), "fourier")
+
+    def antisymmetrize(self, particles):
+        """operation<AntiSymmetrize<particles>> on the X state (wf/grid.h:611-723)."""
+        self.ctx.check(load().qsf_antisymmetrize(self.h, particles), "antisymmetrize")
 
     def evolve(self, mode, rep, delta, coupling=None, couples_in=None):
         c = None
