@@ -97,6 +97,10 @@ int qsf_mask_region(qsf_grid* g);
  * particles 2 swaps the two coordinates of a 2D grid (the last two of a 3D grid), particles 3
  * sums the six permutations of a cubic 3D grid; nothing for dim < particles. */
 int qsf_antisymmetrize(qsf_grid* g, int particles);
+/* croossOut(au_distance) (wf/grid.h:890-907): in place on the X state, keeps nodes with exactly
+ * two coordinates beyond the multiIonizationFilter<2_D, 2_D> radius (:862-888) and ramps the rest
+ * down with 0.5 (1 + cos(pi k / 30)). */
+int qsf_cross_out(qsf_grid* g, double au_distance);
 
 /* ---- propagator: SplitPropagator<MODE, MultiProductSplit<Split3Base<X,P,X>,order>, ...>
  *      (propagator.h:51-499) over a grid set, with the step executed as fused axis passes ---- */

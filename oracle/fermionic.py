@@ -75,3 +75,48 @@ def antisymmetrize_loops(psi, n, dim, particles):
                 p[off(j, i, j)] = 0j
                 p[off(j, j, i)] = 0j
     return np.array(p)
+
+
+# ---- croossOut / multiIonizationFilter<2_D, 2_D> (wf/grid.h:862-907) ----
+RAMP = 30
+
+
+def _neg_dist(i, n):
+    return np.where(i < n // 2, -i, i - n + 1)
+
+
+def cross_out_weights(shape, au_distance=50.0, ramp=RAMP):
+    """The factor croossOut multiplies each node by (vectorised restatement)."""
+    dim = len(shape)
+    from_center = int(au_distance + 2 * ramp)          # (ind) of a double argument
+    from_boundary = shape[0] // 2 - from_center          # n2[0] - nodesFromCenter
+    idx = np.indices(shape)
+    nodes = np.stack([_neg_dist(idx[a], shape[a]) + from_boundary for a in range(dim)])
+    count = (nodes > 0).sum(axis=0)
+    take_max = (count == 0) & (dim != 2)
+    mv = np.where(take_max, nodes.max(axis=0), nodes.min(axis=0))
+    mv = np.minimum(np.abs(mv), ramp)
+    table = np.array([0.5 * (1. + np.cos(np.pi * k / ramp)) for k in range(ramp + 1)])
+    return np.where(count == 2, 1.0, table[mv])
+
+
+def cross_out(psi, shape, au_distance=50.0):
+    return (np.asarray(psi, dtype=np.complex128).reshape(shape) * cross_out_weights(shape, au_distance)).ravel()
+
+
+def multi_ionization_filter_node(nodes_from_center, ramp, nodes, n_axis0, frm=2, to=2):
+    """multiIonizationFilter<from, to>(nodesFromCenter, ramp, nodes...) for one node, with the
+    reference's folds written out (tiny grids)."""
+    import math
+    dim = len(nodes)
+    from_boundary = n_axis0 // 2 - int(nodes_from_center)
+    nodes = [x + from_boundary for x in nodes]
+    count = sum(1 for x in nodes if x > 0)
+    if frm <= count <= to:
+        return 1.0
+    min_val = float(nodes[-1])                          # (nodes, ...): the last one
+    for x in nodes:
+        if (x > min_val) if (count == 0 and frm != dim) else (x < min_val):
+            min_val = x
+    min_val = ramp if abs(min_val) > ramp else abs(min_val)
+    return 0.5 * (1. + math.cos(math.pi * min_val / ramp))

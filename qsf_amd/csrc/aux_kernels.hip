@@ -422,6 +422,52 @@ void launch_antisym(int precision, void* psi, int dim, const int64_t* n, int kin
   }
 }
 
+// croossOut (wf/grid.h:890-907) with multiIonizationFilter<2_D, 2_D> (:862-888): every node is
+// scaled by 1 where exactly two coordinates lie beyond `from_center` nodes from the centre, else
+// by a cos ramp of the integer distance picked by the filter's folds. `tab[k]` =
+// 0.5 * (1 + cos(pi * k / ramp)), k = 0..ramp, is built on the host in FP64.
+template <typename T>
+__global__ void cross_out_kernel(T* psi, int dim, int64_t n0, int64_t n1, int64_t n2, int64_t from_boundary,
+                                 int64_t ramp, const double* tab) {
+  const int64_t n[3] = {n0, n1, n2};
+  const int64_t m = n0 * n1 * n2;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+    int64_t idx[3] = {q / (n1 * n2), (q / n2) % n1, q % n2};
+    int64_t nodes[3];
+    int count = 0;
+    for (int a = 0; a < dim; ++a) {
+      const int64_t i = idx[3 - dim + a], na = n[3 - dim + a];
+      const int64_t nd = i < na / 2 ? -i : i - na + 1;      // neg_dist_from_edge
+      nodes[a] = nd + from_boundary;
+      count += nodes[a] > 0 ? 1 : 0;
+    }
+    if (count == 2) continue;                                // from <= count <= to
+    int64_t mv = nodes[dim - 1];
+    const bool take_max = count == 0 && dim != 2;            // from != DIM
+    for (int a = 0; a < dim; ++a)
+      if (take_max ? nodes[a] > mv : nodes[a] < mv) mv = nodes[a];
+    mv = mv < 0 ? -mv : mv;
+    if (mv > ramp) mv = ramp;
+    const double w = tab[mv];
+    const T x = psi[q];
+    psi[q] = T{(decltype(x.x))((double)x.x * w), (decltype(x.y))((double)x.y * w)};
+  }
+}
+
+// from_boundary = n[0] / 2 - nodesFromCenter (n2[0] of the grid's first axis, :868)
+void launch_cross_out(int precision, void* psi, int dim, const int64_t* n, int64_t from_boundary, int64_t ramp,
+                      const double* tab, hipStream_t s) {
+  const int64_t n0 = dim == 3 ? n[0] : 1, n1 = dim >= 2 ? n[dim - 2] : 1, n2 = n[dim - 1];
+  const int64_t m = n0 * n1 * n2;
+  const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
+  if (precision == QSF_FP64)
+    hipLaunchKernelGGL(cross_out_kernel<double2>, dim3(blocks), dim3(256), 0, s, (double2*)psi, dim, n0, n1, n2,
+                       from_boundary, ramp, tab);
+  else
+    hipLaunchKernelGGL(cross_out_kernel<float2>, dim3(blocks), dim3(256), 0, s, (float2*)psi, dim, n0, n1, n2,
+                       from_boundary, ramp, tab);
+}
+
 void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s) {
   const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
   if (precision == QSF_FP64)

@@ -2018,7 +2018,28 @@ int qsf_flux(qsf_grid* g, const double* vgC, const qsf_border_desc* b, int nb, d
 extern "C++" {
 namespace qsf {
 void launch_antisym(int precision, void* psi, int dim, const int64_t* n, int kind, hipStream_t s);
+void launch_cross_out(int precision, void* psi, int dim, const int64_t* n, int64_t from_boundary, int64_t ramp,
+                      const double* tab, hipStream_t s);
 }
+}
+int qsf_cross_out(qsf_grid* g, double au_distance) {
+  if (!g) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    Grid& G = *g->g;
+    QSF_REQUIRE(G.rep == QSF_REP_X, QSF_ERR_STATE, "croossOut acts on the X representation");
+    const int64_t ramp = 30;
+    // multiIonizationFilter's nodesFromCenter receives au_distance + 2 * ramp as a node count
+    // (grid.h:897-899; the au_distance / dx computed there is unused)
+    const int64_t from_center = (int64_t)(au_distance + 2 * ramp);
+    std::vector<double> tab(ramp + 1);
+    for (int64_t k = 0; k <= ramp; ++k) tab[k] = 0.5 * (1. + std::cos(M_PI * (double)k / (double)ramp));
+    G.aux.ensure(tab.size() * sizeof(double));
+    QSF_HIP(hipMemcpyAsync(G.aux.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, G.st()));
+    launch_cross_out(G.prec, G.psi.p, G.D, G.n, G.n[0] / 2 - from_center, ramp, G.aux.as<double>(), G.st());
+    QSF_HIP(hipGetLastError());
+    QSF_HIP(hipStreamSynchronize(G.st()));   // `tab` is host memory read by the copy
+  });
 }
 int qsf_antisymmetrize(qsf_grid* g, int particles) {
   if (!g || (particles != 2 && particles != 3)) return QSF_ERR_ARG;

@@ -27,6 +27,7 @@ SIGNATURES = {
     "qsf_ctx_query": (_I, [_P, C.POINTER(_I)]),
     "qsf_prop_fast_forward": (_I, [_P, _I64]),
     "qsf_antisymmetrize": (_I, [_P, _I]),
+    "qsf_cross_out": (_I, [_P, C.c_double]),
     "qsf_timer_mark": (_I, [_P]),
     "qsf_timer_elapsed_ms": (_I, [_P, _dp]),
     "qsf_set_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(_P)]),
@@ -261,6 +262,10 @@ class Grid:
     def antisymmetrize(self, particles):
         """operation<AntiSymmetrize<particles>> on the X state (wf/grid.h:611-723)."""
         self.ctx.check(load().qsf_antisymmetrize(self.h, particles), "antisymmetrize")
+
+    def cross_out(self, au_distance=50.0):
+        """croossOut (wf/grid.h:890-907) on the X state."""
+        self.ctx.check(load().qsf_cross_out(self.h, float(au_distance)), "cross_out")
 
     def evolve(self, mode, rep, delta, coupling=None, couples_in=None):
         c = None

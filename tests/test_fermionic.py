@@ -49,3 +49,40 @@ def test_device_antisymmetrize_bit_exact(dim, particles, n):
     assert np.array_equal(gs[0].psi(), F.antisymmetrize(psi, n, dim, particles))
     gs.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("shape,au", [((96, 96), 10.0), ((64, 64), 0.0), ((24, 24, 24), -50.0), ((12, 16, 8), -55.0)])
+def test_cross_out_oracle_equals_filter_folds(shape, au):
+    w = F.cross_out_weights(shape, au)
+    for idx in itertools.product(*[range(n) for n in shape]):
+        nodes = [int(F._neg_dist(np.int64(i), n)) for i, n in zip(idx, shape)]
+        assert w[idx] == F.multi_ionization_filter_node(au + 2 * F.RAMP, F.RAMP, nodes, shape[0])
+    # weights lie in [0, 1]; once the radius fits inside the grid, nodes with exactly two
+    # coordinates beyond it keep weight 1
+    assert w.min() >= 0.0 and w.max() <= 1.0
+    if int(au + 2 * F.RAMP) < min(shape) // 2 - 1:
+        assert w.max() == 1.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,n,au", [(2, 256, 10.0), (2, 192, 0.0), (3, 64, -50.0), (1, 256, 10.0)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_device_cross_out(dim, n, au, prec):
+    from qsf_amd import hip
+    from qsf_amd import qsf_types as T
+    print(f"[case] cross_out dim={dim} n={n} au={au} prec={prec}", flush=True)
+    ctx = hip.Context(0)
+    precision = T.FP64 if prec == 0 else T.FP32_MIXED
+    gs = hip.GridSet(ctx, T.grid_desc(dim, n, 0.3, precision=precision))
+    psi = crand(n ** dim, 6)
+    if precision != T.FP64:
+        psi = psi.astype(np.complex64).astype(np.complex128)
+    gs[0].set_psi(psi)
+    gs[0].cross_out(au)
+    want = F.cross_out(psi, (n,) * dim, au)
+    if precision == T.FP64:
+        assert np.array_equal(gs[0].psi(), want)
+    else:   # the FP64 product rounded once to FP32
+        assert np.array_equal(gs[0].psi(), want.astype(np.complex64).astype(np.complex128))
+    gs.close()
+    ctx.close()
