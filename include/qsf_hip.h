@@ -101,6 +101,10 @@ int qsf_antisymmetrize(qsf_grid* g, int particles);
  * two coordinates beyond the multiIonizationFilter<2_D, 2_D> radius (:862-888) and ramps the rest
  * down with 0.5 (1 + cos(pi k / 30)). */
 int qsf_cross_out(qsf_grid* g, double au_distance);
+/* orthogonalizeWith (wf/grid.h:909-946) with the stored state already read: s = <phi|psi> dV
+ * (FP64), psi -= s * phi; `phi` is count = grid size complex FP64 values (interleaved), s goes
+ * to overlap[2] when given. */
+int qsf_orthogonalize(qsf_grid* g, const double* phi, int64_t count, double* overlap);
 
 /* ---- propagator: SplitPropagator<MODE, MultiProductSplit<Split3Base<X,P,X>,order>, ...>
  *      (propagator.h:51-499) over a grid set, with the step executed as fused axis passes ---- */

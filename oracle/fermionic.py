@@ -120,3 +120,11 @@ def multi_ionization_filter_node(nodes_from_center, ramp, nodes, n_axis0, frm=2,
             min_val = x
     min_val = ramp if abs(min_val) > ramp else abs(min_val)
     return 0.5 * (1. + math.cos(math.pi * min_val / ramp))
+
+
+def orthogonalize(psi, phi, dV):
+    """orthogonalizeWith (wf/grid.h:909-946): s = sum conj(phi) psi * dV, psi - s phi."""
+    psi = np.asarray(psi, dtype=np.complex128).ravel()
+    phi = np.asarray(phi, dtype=np.complex128).ravel()
+    s = np.sum(np.conj(phi) * psi) * dV
+    return psi - s * phi, s

@@ -468,6 +468,74 @@ void launch_cross_out(int precision, void* psi, int dim, const int64_t* n, int64
                        from_boundary, ramp, tab);
 }
 
+// orthogonalizeWith (wf/grid.h:909-946): <phi|psi> = sum conj(phi) psi in FP64 (block partials
+// in a fixed order, then one thread over the partials), then psi -= s * phi with s the overlap
+// times dV. phi is complex FP64 in both precisions.
+template <typename T>
+__global__ void overlap_partials_kernel(const double2* phi, const T* psi, int64_t m, double* partials) {
+  __shared__ double sr[256], si[256];
+  double ar = 0.0, ai = 0.0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = phi[q];
+    const double bx = psi[q].x, by = psi[q].y;
+    ar += a.x * bx + a.y * by;
+    ai += a.x * by - a.y * bx;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sr[0];
+    partials[2 * blockIdx.x + 1] = si[0];
+  }
+}
+__global__ void overlap_finish_kernel(const double* partials, int nb, double* out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double r = 0.0, i = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    r += partials[2 * b];
+    i += partials[2 * b + 1];
+  }
+  out[0] = r;
+  out[1] = i;
+}
+template <typename T>
+__global__ void sub_scaled_kernel(T* psi, const double2* phi, int64_t m, double sr, double si) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = phi[q];
+    const double px = (double)psi[q].x - (sr * a.x - si * a.y);
+    const double py = (double)psi[q].y - (sr * a.y + si * a.x);
+    psi[q] = T{(decltype(psi[q].x))px, (decltype(psi[q].y))py};
+  }
+}
+
+// partials: 2 * 1024 doubles; out: 2 doubles
+void launch_overlap(int precision, const void* phi, const void* psi, int64_t m, double* partials, double* out,
+                    hipStream_t s) {
+  const int nb = (int)((m + 255) / 256 < 1024 ? (m + 255) / 256 : 1024);
+  if (precision == QSF_FP64)
+    hipLaunchKernelGGL(overlap_partials_kernel<double2>, dim3(nb), dim3(256), 0, s, (const double2*)phi,
+                       (const double2*)psi, m, partials);
+  else
+    hipLaunchKernelGGL(overlap_partials_kernel<float2>, dim3(nb), dim3(256), 0, s, (const double2*)phi,
+                       (const float2*)psi, m, partials);
+  hipLaunchKernelGGL(overlap_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)partials, nb, out);
+}
+void launch_sub_scaled(int precision, void* psi, const void* phi, int64_t m, double sr, double si, hipStream_t s) {
+  const int nb = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
+  if (precision == QSF_FP64)
+    hipLaunchKernelGGL(sub_scaled_kernel<double2>, dim3(nb), dim3(256), 0, s, (double2*)psi, (const double2*)phi, m, sr, si);
+  else
+    hipLaunchKernelGGL(sub_scaled_kernel<float2>, dim3(nb), dim3(256), 0, s, (float2*)psi, (const double2*)phi, m, sr, si);
+}
+
 void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s) {
   const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
   if (precision == QSF_FP64)

@@ -2020,7 +2020,35 @@ namespace qsf {
 void launch_antisym(int precision, void* psi, int dim, const int64_t* n, int kind, hipStream_t s);
 void launch_cross_out(int precision, void* psi, int dim, const int64_t* n, int64_t from_boundary, int64_t ramp,
                       const double* tab, hipStream_t s);
+void launch_overlap(int precision, const void* phi, const void* psi, int64_t m, double* partials, double* out,
+                    hipStream_t s);
+void launch_sub_scaled(int precision, void* psi, const void* phi, int64_t m, double sr, double si, hipStream_t s);
 }
+}
+int qsf_orthogonalize(qsf_grid* g, const double* phi, int64_t count, double* overlap) {
+  if (!g || !phi) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] {
+    settle(g);
+    Grid& G = *g->g;
+    QSF_REQUIRE(G.rep == QSF_REP_X, QSF_ERR_STATE, "orthogonalizeWith acts on the X representation");
+    QSF_REQUIRE(count == G.m, QSF_ERR_ARG, "the stored state must have the grid's node count");
+    const size_t phi_bytes = (size_t)G.m * 16;
+    G.aux.ensure(phi_bytes + (2 * 1024 + 2) * sizeof(double));
+    char* base = static_cast<char*>(G.aux.p);
+    double* partials = reinterpret_cast<double*>(base + phi_bytes);
+    double* res = partials + 2 * 1024;
+    QSF_HIP(hipMemcpyAsync(base, phi, phi_bytes, hipMemcpyHostToDevice, G.st()));
+    launch_overlap(G.prec, base, G.psi.p, G.m, partials, res, G.st());
+    QSF_HIP(hipGetLastError());
+    double h[2];
+    QSF_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, G.st()));
+    QSF_HIP(hipStreamSynchronize(G.st()));
+    const double sr = h[0] * G.dV, si = h[1] * G.dV;   // scalarProduct *= vol<X>
+    launch_sub_scaled(G.prec, G.psi.p, base, G.m, sr, si, G.st());
+    QSF_HIP(hipGetLastError());
+    QSF_HIP(hipStreamSynchronize(G.st()));
+    if (overlap) { overlap[0] = sr; overlap[1] = si; }
+  });
 }
 int qsf_cross_out(qsf_grid* g, double au_distance) {
   if (!g) return QSF_ERR_ARG;

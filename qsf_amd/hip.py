@@ -28,6 +28,7 @@ SIGNATURES = {
     "qsf_prop_fast_forward": (_I, [_P, _I64]),
     "qsf_antisymmetrize": (_I, [_P, _I]),
     "qsf_cross_out": (_I, [_P, C.c_double]),
+    "qsf_orthogonalize": (_I, [_P, _dp, _I64, _dp]),
     "qsf_timer_mark": (_I, [_P]),
     "qsf_timer_elapsed_ms": (_I, [_P, _dp]),
     "qsf_set_create": (_I, [_P, C.POINTER(T.GridDesc), C.POINTER(_P)]),
@@ -266,6 +267,13 @@ class Grid:
     def cross_out(self, au_distance=50.0):
         """croossOut (wf/grid.h:890-907) on the X state."""
         self.ctx.check(load().qsf_cross_out(self.h, float(au_distance)), "cross_out")
+
+    def orthogonalize(self, phi):
+        """orthogonalizeWith (wf/grid.h:909-946) against the state `phi`; returns <phi|psi> dV."""
+        a = np.ascontiguousarray(np.asarray(phi, dtype=np.complex128)).ravel()
+        out = np.zeros(2)
+        self.ctx.check(load().qsf_orthogonalize(self.h, _dptr(a), a.size, _dptr(out)), "orthogonalize")
+        return complex(out[0], out[1])
 
     def evolve(self, mode, rep, delta, coupling=None, couples_in=None):
         c = None

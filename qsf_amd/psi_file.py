@@ -173,3 +173,14 @@ def remove_backups(gset, directory="."):
     for name in backup_names(gset, directory) + [os.path.join(str(directory), BACKUP_INFO)]:
         if os.path.exists(name):
             os.remove(name)
+
+
+def orthogonalize_with(gset, name, region=0):
+    """orthogonalizeWith(input_path) (wf/grid.h:909-946): project the state stored in a complex
+    X-representation `.psi` file out of a device grid; returns <phi|psi> dV. The data start after
+    the 11 + 17*dim byte header that `save` writes; the reference reads from 8 + 17*dim bytes
+    (its offset counts no i64 downscale and a 4-byte int), three bytes early."""
+    hd, data = read(name)
+    if hd["rep"] != T.REP_X or not hd["isComplex"] or hd["downscale"] != 1:
+        raise ValueError("orthogonalizeWith needs a complex X-representation file")
+    return gset[region].orthogonalize(np.asarray(data).ravel())

@@ -86,3 +86,45 @@ def test_device_cross_out(dim, n, au, prec):
         assert np.array_equal(gs[0].psi(), want.astype(np.complex64).astype(np.complex128))
     gs.close()
     ctx.close()
+
+
+def test_orthogonalize_oracle_projects_out():
+    n, dx = 32, 0.3
+    dV = dx * dx
+    phi = crand(n * n, 7)
+    phi /= np.sqrt(np.sum(np.abs(phi) ** 2) * dV)      # <phi|phi> dV = 1
+    psi = crand(n * n, 8)
+    out, s = F.orthogonalize(psi, phi, dV)
+    assert abs(np.sum(np.conj(phi) * out) * dV) < 1e-12 * abs(s)
+    # a state already orthogonal is left alone; phi itself is removed completely
+    again, s2 = F.orthogonalize(out, phi, dV)
+    assert abs(s2) < 1e-12 * abs(s) and np.allclose(again, out, rtol=0, atol=1e-13)
+    gone, s3 = F.orthogonalize(2.5 * phi, phi, dV)
+    assert abs(s3 - 2.5) < 1e-13 and np.max(np.abs(gone)) < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [0, 1])
+def test_device_orthogonalize_with_file(tmp_path, prec):
+    from qsf_amd import hip, psi_file as PF
+    from qsf_amd import qsf_types as T
+    n, dx = 96, 0.3
+    print(f"[case] orthogonalize prec={prec}", flush=True)
+    precision = T.FP64 if prec == 0 else T.FP32_MIXED
+    phi = crand(n * n, 9)
+    phi /= np.sqrt(np.sum(np.abs(phi) ** 2) * dx * dx)
+    psi = crand(n * n, 10)
+    if precision != T.FP64:
+        psi = psi.astype(np.complex64).astype(np.complex128)
+    name = PF.write(tmp_path / "ground", phi, [n, n], [dx, dx])
+    ctx = hip.Context(0)
+    gs = hip.GridSet(ctx, T.grid_desc(2, n, dx, precision=precision))
+    gs[0].set_psi(psi)
+    s = PF.orthogonalize_with(gs, name)
+    want, s_ref = F.orthogonalize(psi, phi, dx * dx)
+    tol = 1e-13 if precision == T.FP64 else 1e-6
+    assert abs(s - s_ref) <= 1e-13 * abs(s_ref)
+    got = gs[0].psi()
+    assert np.linalg.norm(got - want) <= tol * np.linalg.norm(want)
+    gs.close()
+    ctx.close()
