@@ -1,7 +1,7 @@
 """Parity of the HIP path (libqsf_hip.so through its C ABI) with the CPU FP64 oracle.
 
-Tolerances: FP64 kernels L2(ψ) relative error ≤ 1e-10 (the bar the north star sets for the FP64
-path; actual errors are ~1e-14); FP32-mixed kernels ≤ 2e-5 relative per operation. Bit-exact
+Tolerances: FP64 kernels L2(ψ) relative error ≤ 1e-10 (the required bar for the FP64 path;
+actual errors are ~1e-14); FP32-mixed kernels ≤ 2e-5 relative per operation. Bit-exact
 operations (region masks, border tables and their indexing) are compared with array_equal.
 """
 import numpy as np
