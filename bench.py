@@ -1,17 +1,17 @@
 """Benchmark of the split-operator step on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on): the 2D two-electron
-reduced model, 2048x2048, dx=0.2, Eberly interaction (Z=2, soft-core 0.5 / 0.33), velocity gauge,
-MultiProductSplit<Split3Base<X,P,X>,2> (the reference's "4th order" split, which returns two
-Strang steps of dt/2), dt=0.02, 800 nm sin^2 pulse of 3 cycles with E0=0.12 on both electrons,
-cos^(1/8) absorber over the outer 10% (205 nodes), multigrid regions NO/X/Y/XY with their masks
-and transfers every step, and FLUX<ZOA_FLUX_2D> plus the norm every step. Synthetic initial state
-(seeded complex Gaussian noise on the main region, zero elsewhere).
+Default workload (the largest single-GPU configuration, where the FP64 roofline target sits):
+the 3D three-electron restricted-geometry model, 512^3, dx=0.3, Eckhardt-Sacha interaction
+(Z=3, soft-core 1.0 / 1.0), length gauge, Strang splitting, dt=0.05, 2-cycle 800 nm sin^2 pulse
+with E0 drawn from U[0.05, 0.15] (seed 3), CAP absorber on the outer 32 nodes, FLUX<BOX<224>>
+(six faces) plus the norm every step. Synthetic initial state (seeded complex Gaussian noise).
 
-One "step" = one loop body of SplitPropagator::run (makeStep + computeEach + postCompute) over
-all four region grids. value = grid-point-steps/s of the main grid (value_all_regions counts
-the four region grids). The CPU baseline is the FP64 oracle on a bounded sample of the same
-workload, timed on this host.
+One "step" = one loop body of SplitPropagator::run (makeStep + computeEach). value =
+grid-point-steps/s in FP64 with the state resident in HBM. The same line carries the FP32-mixed
+form of the same workload (with its error against FP64 after the same steps), the batched
+FP32-mixed step of 64 uneven 2D solves against the plain FP64 step of the same solves, and the
+CPU baseline: the FP64 oracle on the host cores (one whole step of the same workload on all
+cores, and a one-core sample).
 """
 import argparse
 import json
@@ -75,6 +75,11 @@ def c4_solves(precision, count=64, seed=4):
     fields = [T.field_desc(T.AXIS_XY, float(e), 0.057, int(c), phase_in_pi=float(p))
               for e, c, p in zip(e0, cycles, cep)]
     return descs, fields
+
+
+C4_WORKLOAD = ("64 independent 2D two-electron solves, sides U{512..2048}, I log-U[5e13,1e15] W/cm^2, "
+               "CEP U[0,2pi), 2-6 cycles, Strang dt=0.02, velocity gauge, cos^(1/8), norm + ZOA_FLUX_2D "
+               "every step; batched FP32-mixed step replayed as a hipGraph")
 
 
 def c4_prop(order=1):
@@ -298,10 +303,13 @@ def copy_calibration(steps):
             "ms_per_launch": round(ms, 4), "launches": steps + 3}
 
 
-def cpu_baseline(desc, pdesc, fields, budget_s=20.0):
-    """The FP64 oracle (one host core) on whole steps of the same workload, bounded in time."""
+def cpu_baseline(desc, pdesc, fields, budget_s=20.0, threads=0):
+    """The FP64 oracle on whole steps of `desc`, at least one, then until budget_s / 2 (at most
+    50); `threads` OpenMP threads for its independent loops (0: the runtime default, which is
+    OMP_NUM_THREADS where set)."""
     from oracle import oracle as O
-    progress(f"cpu baseline (oracle, one core, about {budget_s / 2:.0f} s)")
+    used = O.set_threads(threads)
+    progress(f"cpu baseline: oracle on {used} thread(s), grid {list(desc.n)[:desc.dim]}")
     gs = O.GridSet(desc)
     gs[0].set_psi(initial_state(gs[0].m))
     prop = O.Propagator(gs, pdesc, fields)
@@ -312,11 +320,58 @@ def cpu_baseline(desc, pdesc, fields, budget_s=20.0):
         prop.step()
         steps += 1
     dt = time.perf_counter() - t0
-    m_main = gs[0].m
-    return {"value": round(m_main * steps / dt, 1), "unit": "grid-point-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{steps} whole step(s) of the same workload (all {len(gs)} region grids, "
-                      f"observables every step) on one host core in {dt:.1f} s"}
+    return dict(value=round(gs[0].m * steps / dt, 1), cores=used, steps=steps, seconds=round(dt, 2),
+                m=gs[0].m, regions=len(gs))
+
+
+def host_cores():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"nproc": os.cpu_count(), "available": avail, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline_c3(pdesc, fields):
+    """All threads on one whole step of the 512^3 workload, and one thread on one whole step of
+    the same physics at 256^3 (a 512^3 step takes about 90 s on one core)."""
+    full = cpu_baseline(c3_desc(T.FP64), pdesc, fields, budget_s=0.0, threads=0)
+    one = cpu_baseline(c3_desc(T.FP64, n=256), pdesc, fields, budget_s=0.0, threads=1)
+    return {"value": full["value"], "unit": "grid-point-steps/s", "cores": full["cores"], "kind": "port",
+            "sample": f"{full['steps']} whole step(s) of the 512^3 workload (observables every step) "
+                      f"on {full['cores']} host threads in {full['seconds']} s",
+            "host": host_cores(),
+            "single_core": {"value": one["value"], "unit": "grid-point-steps/s", "cores": 1,
+                            "sample": f"{one['steps']} whole step(s) of the same physics at 256^3 on one "
+                                      f"host thread in {one['seconds']} s"}}
+
+
+def cpu_baseline_c2(desc, pdesc, fields):
+    full = cpu_baseline(desc, pdesc, fields, threads=0)
+    one = cpu_baseline(desc, pdesc, fields, threads=1)
+    return {"value": full["value"], "unit": "grid-point-steps/s", "cores": full["cores"], "kind": "port",
+            "sample": f"{full['steps']} whole step(s) of the same workload (all {full['regions']} region "
+                      f"grids, observables every step) on {full['cores']} host threads in {full['seconds']} s",
+            "host": host_cores(),
+            "single_core": {"value": one["value"], "unit": "grid-point-steps/s", "cores": 1,
+                            "sample": f"{one['steps']} whole step(s) on one host thread in {one['seconds']} s"}}
+
+
+FP32_BAR = 1e-6
+
+
+def fp32_vs_fp64(m, r, nb):
+    """Errors of the FP32-mixed run `m` against the FP64 run `r` after the same steps."""
+    a, b = m["records"], r["records"]
+    fa = a[:, T.REC_FLUX0:T.REC_FLUX0 + nb]
+    fb = b[:, T.REC_FLUX0:T.REC_FLUX0 + nb]
+    errs = {
+        "steps": int(b[-1, T.REC_STEP]),
+        "norm_rel_err": float(np.max(np.abs(a[:, T.REC_NORM] / b[:, T.REC_NORM] - 1))),
+        "flux_max_rel_err": float(np.abs(fa - fb).max() / max(np.abs(fb).max(), 1e-300)),
+        "psi_l2_rel_err": float(np.linalg.norm(m["psi"] - r["psi"]) / np.linalg.norm(r["psi"]))}
+    errs["meets_1e-6_bar"] = all(errs[k] <= FP32_BAR for k in ("norm_rel_err", "flux_max_rel_err", "psi_l2_rel_err"))
+    return errs
 
 
 def main():
@@ -324,10 +379,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c2")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c3")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="c3: leave out the batched 2D comparison")
     ap.add_argument("--requests", type=int, default=40, help="c5: number of requests in the stream")
-    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="leave out the FP32-mixed run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every solve lives whole on one MI355X; --gpus {args.gpus} "
@@ -360,11 +416,7 @@ def main():
                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "c64 (f32, f64 phases and reductions)", "data": "synthetic",
-               "config": {"workload": "64 independent 2D two-electron solves, sides U{512..2048}, "
-                                      "I log-U[5e13,1e15] W/cm^2, CEP U[0,2pi), 2-6 cycles, Strang dt=0.02, "
-                                      "velocity gauge, cos^(1/8), norm + ZOA_FLUX_2D every step; batched "
-                                      "FP32-mixed step replayed as a hipGraph",
-                          "grid_points_per_step": r["pts"], "solves_per_side": r["sides"]},
+               "config": {"workload": C4_WORKLOAD, "grid_points_per_step": r["pts"], "solves_per_side": r["sides"]},
                "plain_fp64": {"value": round(r["plain"], 1), "ms_per_step": round(r["plain_ms_per_step"], 4),
                               "per_size_ms_per_step": r["per_size_ms"]},
                "speedup_vs_plain_fp64": round(r["value"] / r["plain"], 3),
@@ -377,11 +429,13 @@ def main():
                     "MultiProductSplit order 2, dt=0.02, 3-cycle 800 nm sin^2 E0=0.12, cos^(1/8) "
                     "absorber, multigrid NO/X/Y/XY, ZOA_FLUX_2D + norm every step")
         pts_bytes = 64 * 2   # two P substeps per step survive (last MPS chain), 2D FP64
+        nb = len(T.ZOA_FLUX_2D)
     else:
         desc, pdesc, fields = c3_desc(T.FP64), c3_prop(), c3_fields()
         workload = ("3D three-electron Eckhardt-Sacha 512^3 dx=0.3, Z=3, length gauge, Strang, "
                     "dt=0.05, 2-cycle 800 nm, CAP 32 nodes, BOX<224> flux + norm every step")
         pts_bytes = 128
+        nb = 1
 
     r = run_device(desc, pdesc, fields, args.steps, args.warmup, keep_state=True)
     ms_per_step = r["wall"] / r["done"] * 1e3
@@ -392,31 +446,34 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128 (f64)",
         "data": "synthetic", "config": {"workload": workload, "grid_points_main": r["m_main"],
                                         "grid_points_all_regions": r["m_all"]},
-        "value_all_regions": round(r["m_all"] * r["done"] / r["wall"], 1),
         "device_ms_per_step": round(r["dev_ms"] / r["done"], 4),
         "step_roofline": {"bytes_per_grid_point_step": pts_bytes,
                           "frac": round(value * pts_bytes / (HBM_PEAK_GBS * 1e9), 4)},
         "roofline": roofline_of(r["kernels"]),
         "kernels": {k: {"launches": v[0], "avg_us": round(v[1] / v[0] * 1e3, 2)} for k, v in r["kernels"].items()},
     }
+    if args.workload == "c2":
+        out["value_all_regions"] = round(r["m_all"] * r["done"] / r["wall"], 1)
+    r["kernels"] = None
     if not args.no_variants:
-        if args.workload == "c2":
-            m = run_device(c2_desc(T.FP32_MIXED), pdesc, fields, args.steps, args.warmup, profile=False,
-                           keep_state=True)
-            a, b = m["records"], r["records"]
-            nb = len(T.ZOA_FLUX_2D)
-            flux_err = np.abs(a[:, T.REC_FLUX0:T.REC_FLUX0 + nb] - b[:, T.REC_FLUX0:T.REC_FLUX0 + nb]).max() / \
-                max(np.abs(b[:, T.REC_FLUX0:T.REC_FLUX0 + nb]).max(), 1e-300)
-            out["fp32_mixed"] = {
-                "value": round(m["m_main"] * m["done"] / m["wall"], 1),
-                "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4),
-                "vs_fp64_same_steps": {
-                    "steps": int(b[-1, T.REC_STEP]),
-                    "norm_rel_err": float(abs(a[-1, T.REC_NORM] - b[-1, T.REC_NORM]) / abs(b[-1, T.REC_NORM])),
-                    "flux_max_rel_err": float(flux_err),
-                    "psi_l2_rel_err": float(np.linalg.norm(m["psi"] - r["psi"]) / np.linalg.norm(r["psi"]))}}
+        d32 = c2_desc(T.FP32_MIXED) if args.workload == "c2" else c3_desc(T.FP32_MIXED)
+        m = run_device(d32, pdesc, fields, args.steps, args.warmup, profile=False, keep_state=True)
+        out["fp32_mixed"] = {"value": round(m["m_main"] * m["done"] / m["wall"], 1),
+                             "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4),
+                             "vs_fp64_same_steps": fp32_vs_fp64(m, r, nb)}
+        del m
+    r["psi"] = None
+    if args.workload == "c3" and not args.no_c4:
+        c4 = run_c4(args.steps, args.warmup)
+        out["batched_fp32_2d"] = {
+            "workload": C4_WORKLOAD, "value": round(c4["value"], 1), "unit": "grid-point-steps/s",
+            "ms_per_step": round(c4["ms_per_step"], 4),
+            "plain_fp64_value": round(c4["plain"], 1), "plain_fp64_ms_per_step": round(c4["plain_ms_per_step"], 4),
+            "speedup_vs_plain_fp64": round(c4["value"] / c4["plain"], 3),
+            "vs_fp64_same_steps": c4["vs_fp64"]}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(desc, pdesc, fields)
+        out["cpu_baseline"] = cpu_baseline_c3(pdesc, fields) if args.workload == "c3" else \
+            cpu_baseline_c2(desc, pdesc, fields)
     print(json.dumps(out))
 
 

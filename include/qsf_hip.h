@@ -9,7 +9,9 @@
  * (qsf_last_error(ctx) returns the message); handles are owned by the library; host buffers are
  * owned by the caller; wave functions cross the boundary as interleaved FP64 (re, im) in the
  * reference's row-major layout (X rep) or FFTW's transposed P layout (D>1 main grids,
- * wf/grid.h:238-248). One HIP stream per context; calls on one context are not thread-safe.
+ * wf/grid.h:238-248). One HIP stream per context; calls on one context are not thread-safe,
+ * different contexts may be used from different threads (the library keeps no process-wide
+ * mutable state).
  */
 #ifndef QSF_HIP_H
 #define QSF_HIP_H
@@ -66,6 +68,8 @@ int qsf_tune_set(qsf_ctx* ctx, int precision, int row, int64_t n, int64_t grid_s
 /* ---- grid sets: one grid, or the 2^D regions of MultiRegionsReduced (mpi_logic.h:246-263) ---- */
 /* Replaces LocalGrid(Section&) construction (grid.h:115-123, multigrid.h:45-46, 124-202). */
 int qsf_set_create(qsf_ctx* ctx, const qsf_grid_desc* desc, qsf_set** out);
+/* Sets and the propagators created on them may be destroyed in either order; a propagator whose
+ * set is gone answers its scalar queries and qsf_prop_destroy, and QSF_ERR_STATE otherwise. */
 int qsf_set_destroy(qsf_set* set);
 int qsf_set_count(const qsf_set* set, int* count);
 int qsf_set_grid(qsf_set* set, int region, qsf_grid** out);

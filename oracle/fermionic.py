@@ -7,14 +7,23 @@ reference's loops node by node (pure Python, tiny grids) and pins it.
 import numpy as np
 
 
-def antisymmetrize(psi, n, dim, particles):
-    """psi: complex array of n^dim nodes, row-major. Returns the new array (input untouched)."""
-    a = np.array(psi, dtype=np.complex128).reshape([n] * dim)
+def _shape(n, dim):
+    return tuple(int(v) for v in n) if hasattr(n, "__len__") else (int(n),) * dim
+
+
+def antisymmetrize(psi, n, dim, particles, dtype=np.complex128):
+    """psi: complex array of prod(shape) nodes, row-major; `n` is the axis length or the shape
+    (AntiSymmetrize<2_D> on a 3D grid allows n0 != n1 = n2). Returns the new array (input
+    untouched), computed in `dtype` (complex64: every addition a float32 operation, in the
+    order below)."""
+    shape = _shape(n, dim)
+    a = np.array(psi, dtype=dtype).reshape(shape)
     if dim < particles:
         return a.ravel()
     if particles == 2 and dim == 2:                     # :700-716 (psi_total = original)
         return (a - a.T).ravel()
     if particles == 2 and dim == 3:                     # :686-699, last two axes
+        n = shape[1]
         out = a - a.transpose(0, 2, 1)
         j, k = np.indices((n, n))
         lower = k < j
@@ -24,6 +33,7 @@ def antisymmetrize(psi, n, dim, particles):
         out[:, np.arange(n), np.arange(n)] = 0
         return out.ravel()
     # particles == 3, dim == 3 (:646-677)
+    n = shape[0]
     t = a.transpose(1, 2, 0) + a.transpose(2, 0, 1)     # (j,k,i) + (k,i,j) seen from (i,j,k)
     t = t - a.transpose(1, 0, 2)
     t = t - a.transpose(0, 2, 1)
@@ -43,7 +53,9 @@ def antisymmetrize(psi, n, dim, particles):
 def antisymmetrize_loops(psi, n, dim, particles):
     """The reference's loops, node by node (tiny grids only)."""
     p = [complex(x) for x in np.asarray(psi).ravel()]
-    off = (lambda i, j: i * n + j) if dim == 2 else (lambda i, j, k: (i * n + j) * n + k)
+    shape = _shape(n, dim)
+    n = shape[-1]
+    off = (lambda i, j: i * n + j) if dim == 2 else (lambda i, j, k: (i * shape[1] + j) * n + k)
     if dim < particles:
         return np.array(p)
     if particles == 2 and dim == 2:
@@ -52,8 +64,8 @@ def antisymmetrize_loops(psi, n, dim, particles):
             for j in range(n):
                 p[off(i, j)] -= total[off(j, i)]
     elif particles == 2:
-        for i in range(n):
-            for j in range(n):
+        for i in range(shape[0]):
+            for j in range(shape[1]):
                 for k in range(j):
                     p[off(i, j, k)] -= p[off(i, k, j)]
                     p[off(i, k, j)] = -p[off(i, j, k)]

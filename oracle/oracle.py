@@ -26,6 +26,7 @@ def lib():
         sig = {
             "orc_fft1d": (None, [I64, I, dp, dp]),
             "orc_dft_bruteforce": (None, [I64, I, dp, dp]),
+            "orc_set_threads": (I, [I]),
             "orc_set_create": (P, [C.POINTER(T.GridDesc)]),
             "orc_set_destroy": (None, [P]),
             "orc_set_count": (I, [P]),
@@ -86,6 +87,12 @@ def fft1d(x, sign=-1):
     out = np.empty_like(x)
     lib().orc_fft1d(x.size, sign, _dp(x), _dp(out))
     return out
+
+
+def set_threads(k):
+    """OpenMP threads of the oracle's independent loops (FFT lines, pointwise phases); sums stay
+    serial, so every result is the same at any thread count. Returns the count in use."""
+    return lib().orc_set_threads(int(k))
 
 
 def dft_bruteforce(x, sign=-1):

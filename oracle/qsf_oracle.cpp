@@ -20,6 +20,8 @@
 #include <memory>
 #include <vector>
 
+#include <omp.h>
+
 namespace orc {
 
 using cxd = std::complex<double>;
@@ -125,14 +127,19 @@ static void fft_axis(cxd* a, const ind* sh, int axis, int sign) {
   for (int d = axis + 1; d < 3; ++d) inner *= sh[d];
   ind outer = 1;
   for (int d = 0; d < axis; ++d) outer *= sh[d];
-  std::vector<cxd> line(n);
-  for (ind o = 0; o < outer; ++o)
-    for (ind i = 0; i < inner; ++i) {
+  // lines are independent: threads split them, each line's arithmetic is unchanged
+#pragma omp parallel
+  {
+    std::vector<cxd> line(n);
+#pragma omp for schedule(static)
+    for (ind q = 0; q < outer * inner; ++q) {
+      const ind o = q / inner, i = q % inner;
       cxd* base = a + o * n * inner + i;
       for (ind k = 0; k < n; ++k) line[k] = base[k * inner];
       f.exec(line.data(), sign);
       for (ind k = 0; k < n; ++k) base[k * inner] = line[k];
     }
+  }
 }
 
 // Swap axes 0 and 1 of a row-major (n0, n1, n2) array -> (n1, n0, n2).
@@ -569,7 +576,8 @@ struct Grid {
       } else {
         if (transposedP()) swap01(psi, n[1], n[0], n[2]);
         for (int a = 0; a < D; ++a) fft_axis(psi.data(), sh, a, +1);
-        for (auto& v : psi) v *= inv_m;  // normalizeAfterTwoFFT grid.h:363-367
+#pragma omp parallel for schedule(static)
+        for (ind c = 0; c < m; ++c) psi[c] *= inv_m;  // normalizeAfterTwoFFT grid.h:363-367
       }
     } else {
       int boundedCount = 0;
@@ -588,22 +596,22 @@ struct Grid {
   void evolve(int M, int R, double delta, const double* C, int couplesIn) {
     ind s[3];
     shape(R, s);
-    ind c = 0;
-    double co[3] = {0, 0, 0};
-    for (ind i0 = 0; i0 < s[0]; ++i0)
-      for (ind i1 = 0; i1 < s[1]; ++i1)
-        for (ind i2 = 0; i2 < s[2]; ++i2, ++c) {
-          ind idx[3] = {i0, i1, i2};
-          for (int a = 0; a < D; ++a) co[a] = pos(R, a, idx[a]);
-          double v = delta * H(R, co, C, couplesIn);
-          if (M == QSF_MODE_IM) psi[c] *= std::exp(-v);
-          else psi[c] *= cxd{std::cos(-v), std::sin(-v)};
-          if (d.absorber != QSF_ABS_NONE && R == QSF_REP_X && !many) {
-            ind nd[3];
-            for (int a = 0; a < D; ++a) nd[a] = neg_dist_from_edge(a, idx[a]);
-            psi[c] *= absorb(delta, D, nd);
-          }
-        }
+    // row-major odometer of grid.h:414-417, one independent factor per point
+#pragma omp parallel for schedule(static)
+    for (ind c = 0; c < s[0] * s[1] * s[2]; ++c) {
+      const ind i0 = c / (s[1] * s[2]), i1 = (c / s[2]) % s[1], i2 = c % s[2];
+      double co[3] = {0, 0, 0};
+      ind idx[3] = {i0, i1, i2};
+      for (int a = 0; a < D; ++a) co[a] = pos(R, a, idx[a]);
+      double v = delta * H(R, co, C, couplesIn);
+      if (M == QSF_MODE_IM) psi[c] *= std::exp(-v);
+      else psi[c] *= cxd{std::cos(-v), std::sin(-v)};
+      if (d.absorber != QSF_ABS_NONE && R == QSF_REP_X && !many) {
+        ind nd[3];
+        for (int a = 0; a < D; ++a) nd[a] = neg_dist_from_edge(a, idx[a]);
+        psi[c] *= absorb(delta, D, nd);
+      }
+    }
   }
 
   // grid.h:425-441 with Spin0::call (hamiltonian.h:81-90)
@@ -641,6 +649,7 @@ struct Grid {
   // grid.h:467-528 on one rank, then velocityGaugeCorrection (propagator.h:249-257)
   void current_map(const double* vgC) {
     curr.assign(m, {0, 0, 0});
+#pragma omp parallel for schedule(static)
     for (ind c = 0; c < m; ++c)
       for (int dir = 0; dir < D; ++dir) {
         ind s = strideX(dir);
@@ -1102,6 +1111,12 @@ void orc_fft1d(int64_t n, int sign, const double* in, double* out) {
   FFT1D f(n);
   f.exec(a.data(), sign);
   for (int64_t i = 0; i < n; ++i) { out[2 * i] = a[i].real(); out[2 * i + 1] = a[i].imag(); }
+}
+
+// OpenMP threads of the parallel loops (<= 0: the runtime's default); returns the count in use.
+int orc_set_threads(int k) {
+  if (k > 0) omp_set_num_threads(k);
+  return omp_get_max_threads();
 }
 
 void orc_dft_bruteforce(int64_t n, int sign, const double* in, double* out) {

@@ -21,6 +21,7 @@ typedef struct orc_prop orc_prop;
 
 /* FFT pins: own mixed-radix FFT, and a long-double brute-force DFT (sign -1 = forward). */
 void orc_fft1d(int64_t n, int sign, const double* in, double* out);
+int orc_set_threads(int k);
 void orc_dft_bruteforce(int64_t n, int sign, const double* in, double* out);
 
 /* Grid set: one grid (multigrid=0) or the 2^D regions of MultiRegionsReduced (multigrid=1). */

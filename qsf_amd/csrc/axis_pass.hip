@@ -1,45 +1,70 @@
-// axis_pass.hip — public launchers of the fused axis pass (kernels in axis_pass_impl.hpp,
-// instantiated per precision and mode in axis_pass_{f64,f32}_{row,col}.hip).
+// axis_pass.hip — public launchers of the fused axis pass. The kernels live in
+// axis_pass_impl.hpp and are instantiated per (precision, mode, length group) by
+// axis_pass_inst.hip; this file only dispatches on the runtime axis length.
 #include <hip/hip_runtime.h>
 
-#include "qsf_launch.hpp"
+#include "axis_pass_impl.hpp"
 
 namespace qsf {
 
-int dispatch_f64_row(int64_t n, int v, const PassParams& p, hipStream_t s);
-int dispatch_f64_col(int64_t n, int v, const PassParams& p, hipStream_t s);
-int dispatch_f32_row(int64_t n, int v, const PassParams& p, hipStream_t s);
-int dispatch_f32_col(int64_t n, int v, const PassParams& p, hipStream_t s);
-int64_t tiles_f64_row(int64_t n, int v, const PassParams& p);
-int64_t tiles_f64_col(int64_t n, int v, const PassParams& p);
-int64_t tiles_f32_row(int64_t n, int v, const PassParams& p);
-int64_t tiles_f32_col(int64_t n, int v, const PassParams& p);
+#define QSF_EXTERN_ONE(R, ROW, NN)                                                   \
+  extern template int launch_size<R, ROW, NN>(int, const PassParams&, hipStream_t); \
+  extern template int64_t tiles_size<R, ROW, NN>(int, const PassParams&);
+#define QSF_EXTERN_D_ROW(NN) QSF_EXTERN_ONE(double, true, NN)
+#define QSF_EXTERN_D_COL(NN) QSF_EXTERN_ONE(double, false, NN)
+#define QSF_EXTERN_F_ROW(NN) QSF_EXTERN_ONE(float, true, NN)
+#define QSF_EXTERN_F_COL(NN) QSF_EXTERN_ONE(float, false, NN)
+QSF_SIZES(QSF_EXTERN_D_ROW)
+QSF_SIZES(QSF_EXTERN_D_COL)
+QSF_SIZES(QSF_EXTERN_F_ROW)
+QSF_SIZES(QSF_EXTERN_F_COL)
+
+template <typename Real, bool ROW>
+static int dispatch(int64_t n, int v, const PassParams& p, hipStream_t s) {
+  switch (n) {
+#define QSF_CASE(NN) case NN: return launch_size<Real, ROW, NN>(v, p, s);
+    QSF_SIZES(QSF_CASE)
+#undef QSF_CASE
+    default: return -1;
+  }
+}
+
+template <typename Real, bool ROW>
+static int64_t tiles(int64_t n, int v, const PassParams& p) {
+  switch (n) {
+#define QSF_CASE(NN) case NN: return tiles_size<Real, ROW, NN>(v, p);
+    QSF_SIZES(QSF_CASE)
+#undef QSF_CASE
+    default: return -1;
+  }
+}
 
 bool axis_size_supported(int64_t n) {
   switch (n) {
-    case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024: case 2048:
-    case 4096: case 12: case 24: case 48: case 96: case 192: case 384: case 768: case 1536:
-    case 3072:
+#define QSF_CASE(NN) case NN:
+    QSF_SIZES(QSF_CASE)
+#undef QSF_CASE
       return true;
     default: return false;
   }
 }
 
-int axis_variants(int64_t n) { return (n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 ||
-                                        n == 768 || n == 1536) ? 6 : 1; }
+int axis_variants(int64_t n) { return tuned_length(n) ? kVariants : 1; }
 
 int launch_axis_pass(int precision, const PassParams& p, hipStream_t s, int v) {
   const int64_t n = p.n[p.axis];
   const bool row = (p.inner == 1);
-  if (precision == 0) return row ? dispatch_f64_row(n, v, p, s) : dispatch_f64_col(n, v, p, s);
-  return row ? dispatch_f32_row(n, v, p, s) : dispatch_f32_col(n, v, p, s);
+  if (v < 0 || v >= axis_variants(n)) v = 0;
+  if (precision == 0) return row ? dispatch<double, true>(n, v, p, s) : dispatch<double, false>(n, v, p, s);
+  return row ? dispatch<float, true>(n, v, p, s) : dispatch<float, false>(n, v, p, s);
 }
 
 int64_t axis_pass_tiles(int precision, const PassParams& p, int v) {
   const int64_t n = p.n[p.axis];
   const bool row = (p.inner == 1);
-  if (precision == 0) return row ? tiles_f64_row(n, v, p) : tiles_f64_col(n, v, p);
-  return row ? tiles_f32_row(n, v, p) : tiles_f32_col(n, v, p);
+  if (v < 0 || v >= axis_variants(n)) v = 0;
+  if (precision == 0) return row ? tiles<double, true>(n, v, p) : tiles<double, false>(n, v, p);
+  return row ? tiles<float, true>(n, v, p) : tiles<float, false>(n, v, p);
 }
 
 }  // namespace qsf

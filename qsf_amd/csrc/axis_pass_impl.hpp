@@ -521,14 +521,15 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
   const size_t red_bytes = 16 * 3 * sizeof(double);
   const size_t xchg_bytes = (size_t)G::lds_cap(G::W);
   const size_t gather_bytes = (size_t)G::W * N * G::CB;
-  static bool attr_set = false;
-  if (!attr_set) {
+  // once per kernel (a function-local static initialises once, also under concurrent callers)
+  static const bool attr_set = [&] {
     size_t most = red_bytes > xchg_bytes ? red_bytes : xchg_bytes;
     if (ROW && gather_bytes > most) most = gather_bytes;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
-    attr_set = true;
-  }
+    return true;
+  }();
+  (void)attr_set;
   size_t lds = red_bytes;
   if ((p.flags & (F_INV1 | F_FWD | F_INV2)) && xchg_bytes > lds) lds = xchg_bytes;
   if (ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && gather_bytes > lds) lds = gather_bytes;
@@ -545,13 +546,17 @@ inline int64_t tiles_one(const PassParams& p) {
 }
 
 // Lengths with tuned variants; every other supported length has variant 0 only.
-inline bool has_variants(int64_t n) {
+constexpr bool tuned_length(int64_t n) {
   return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 768 || n == 1536;
 }
+inline bool has_variants(int64_t n) { return tuned_length(n); }
 
+// Launcher and tile count of one axis length (all its variants). Instantiated explicitly per
+// (precision, mode, length group) in axis_pass_inst.hip, so the heavy kernels compile in
+// parallel translation units; axis_pass.hip dispatches on the runtime length.
 template <typename Real, bool ROW, int N>
-inline int launch_v(int v, const PassParams& p, hipStream_t s) {
-  if constexpr (N == 256 || N == 512 || N == 1024 || N == 2048 || N == 4096 || N == 768 || N == 1536) {
+int launch_size(int v, const PassParams& p, hipStream_t s) {
+  if constexpr (tuned_length(N)) {
     switch (v) {
       case 1: return launch_one<Real, N, ROW, 1>(p, s);
       case 2: return launch_one<Real, N, ROW, 2>(p, s);
@@ -565,8 +570,8 @@ inline int launch_v(int v, const PassParams& p, hipStream_t s) {
 }
 
 template <typename Real, bool ROW, int N>
-inline int64_t tiles_v(int v, const PassParams& p) {
-  if constexpr (N == 256 || N == 512 || N == 1024 || N == 2048 || N == 4096 || N == 768 || N == 1536) {
+int64_t tiles_size(int v, const PassParams& p) {
+  if constexpr (tuned_length(N)) {
     switch (v) {
       case 1: return tiles_one<Real, N, ROW, 1>(p);
       case 2: return tiles_one<Real, N, ROW, 2>(p);
@@ -581,25 +586,5 @@ inline int64_t tiles_v(int v, const PassParams& p) {
 
 #define QSF_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) \
   X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072)
-
-template <typename Real, bool ROW>
-inline int dispatch_n(int64_t n, int v, const PassParams& p, hipStream_t s) {
-  switch (n) {
-#define QSF_CASE(NN) case NN: return launch_v<Real, ROW, NN>(v, p, s);
-    QSF_SIZES(QSF_CASE)
-#undef QSF_CASE
-    default: return -1;
-  }
-}
-
-template <typename Real, bool ROW>
-inline int64_t tiles_n(int64_t n, int v, const PassParams& p) {
-  switch (n) {
-#define QSF_CASE(NN) case NN: return tiles_v<Real, ROW, NN>(v, p);
-    QSF_SIZES(QSF_CASE)
-#undef QSF_CASE
-    default: return -1;
-  }
-}
 
 }  // namespace qsf

@@ -1,0 +1,47 @@
+// axis_pass_inst.hip — explicit instantiations of the fused axis pass for one precision, one
+// mode and one group of axis lengths. The Makefile compiles this file once per
+// (QSF_PREC: 0 = double, 1 = float; QSF_ROWMODE: 1 = row, 0 = column; QSF_GROUP: 0..9), so the
+// kernels of the long transforms build in parallel translation units.
+#include "axis_pass_impl.hpp"
+
+#if !defined(QSF_PREC) || !defined(QSF_ROWMODE) || !defined(QSF_GROUP)
+#error "QSF_PREC, QSF_ROWMODE and QSF_GROUP must be defined"
+#endif
+
+#if QSF_PREC == 0
+using QsfReal = double;
+#else
+using QsfReal = float;
+#endif
+
+#if QSF_GROUP == 0
+#define QSF_GROUP_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(12) X(24) X(48) X(96)
+#elif QSF_GROUP == 1
+#define QSF_GROUP_SIZES(X) X(192) X(384)
+#elif QSF_GROUP == 2
+#define QSF_GROUP_SIZES(X) X(3072)
+#elif QSF_GROUP == 3
+#define QSF_GROUP_SIZES(X) X(256)
+#elif QSF_GROUP == 4
+#define QSF_GROUP_SIZES(X) X(512)
+#elif QSF_GROUP == 5
+#define QSF_GROUP_SIZES(X) X(768)
+#elif QSF_GROUP == 6
+#define QSF_GROUP_SIZES(X) X(1024)
+#elif QSF_GROUP == 7
+#define QSF_GROUP_SIZES(X) X(1536)
+#elif QSF_GROUP == 8
+#define QSF_GROUP_SIZES(X) X(2048)
+#elif QSF_GROUP == 9
+#define QSF_GROUP_SIZES(X) X(4096)
+#else
+#error "QSF_GROUP out of range"
+#endif
+
+namespace qsf {
+#define QSF_INST(NN)                                                                            \
+  template int launch_size<QsfReal, (QSF_ROWMODE != 0), NN>(int, const PassParams&, hipStream_t); \
+  template int64_t tiles_size<QsfReal, (QSF_ROWMODE != 0), NN>(int, const PassParams&);
+QSF_GROUP_SIZES(QSF_INST)
+#undef QSF_INST
+}  // namespace qsf

@@ -517,9 +517,13 @@ struct Grid {
     aux.ensure(m * cb);
     QSF_HIP(hipMemcpyAsync(aux.p, psi.p, m * cb, hipMemcpyDeviceToDevice, st()));
     const double C[3] = {0.01, 0.01, 0.01};
+    // the passes of this grid's fused step; passes that share a tuning key (same mode and
+    // length: the two column passes of a cubic 3D grid) are timed together per variant
     struct Cand { int axis; uint32_t flags; };
     std::vector<Cand> cands;
-    if (fftAxes.size() >= 2) {
+    if (D == 3 && fftAxes.size() == 3) {
+      for (int a = D - 1; a >= 0; --a) cands.push_back({a, F_FWD | F_P | F_INV2 | (a == D - 1 ? F_XA | F_XB : 0u)});
+    } else if (fftAxes.size() >= 2) {
       cands.push_back({D - 1, F_INV1 | F_XA | F_XB | F_FWD});
       cands.push_back({0, F_FWD | F_P | F_INV2});
       for (int a = 1; a <= D - 2; ++a) cands.push_back({a, F_FWD});
@@ -529,28 +533,34 @@ struct Grid {
     hipEvent_t e0, e1;
     QSF_HIP(hipEventCreate(&e0));
     QSF_HIP(hipEventCreate(&e1));
-    for (const Cand& c : cands) {
-      const int row = (c.axis == D - 1);
-      const int nv = axis_variants(n[c.axis]);
+    std::map<std::pair<int, int64_t>, std::vector<Cand>> keyed;   // (row, n) -> passes
+    for (const Cand& c : cands) keyed[{c.axis == D - 1 ? 1 : 0, n[c.axis]}].push_back(c);
+    for (const auto& kv : keyed) {
+      const int row = kv.first.first;
+      const int64_t len = kv.first.second;
+      const int nv = axis_variants(len);
       int best = 0;
       double best_ms = 1e300;
       for (int v = 0; v < nv; ++v) {
-        PassParams p = params(c.axis, aux.p, aux.p);
-        p.flags = c.flags;
-        if (c.flags & F_XA) p.xa = x_factor(QSF_MODE_RE, 0.01, C, true);
-        if (c.flags & F_XB) p.xb = x_factor(QSF_MODE_RE, 0.01, C, true);
-        if (c.flags & F_P) p_factor(QSF_MODE_RE, 0.01, C, true, true, p.pf);
-        run(p, v);  // warm-up (and table build) outside the timed region
-        QSF_HIP(hipEventRecord(e0, st()));
-        for (int r = 0; r < reps; ++r) run(p, v);
-        QSF_HIP(hipEventRecord(e1, st()));
-        QSF_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        QSF_HIP(hipEventElapsedTime(&ms, e0, e1));
-        const double per = ms / reps;
+        double per = 0.0;
+        for (const Cand& c : kv.second) {
+          PassParams p = params(c.axis, aux.p, aux.p);
+          p.flags = c.flags;
+          if (c.flags & F_XA) p.xa = x_factor(QSF_MODE_RE, 0.01, C, true);
+          if (c.flags & F_XB) p.xb = x_factor(QSF_MODE_RE, 0.01, C, true);
+          if (c.flags & F_P) p_factor(QSF_MODE_RE, 0.01, C, true, true, p.pf);
+          run(p, v);  // warm-up (and table build) outside the timed region
+          QSF_HIP(hipEventRecord(e0, st()));
+          for (int r = 0; r < reps; ++r) run(p, v);
+          QSF_HIP(hipEventRecord(e1, st()));
+          QSF_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          QSF_HIP(hipEventElapsedTime(&ms, e0, e1));
+          per += ms / reps;
+        }
         if (per < best_ms) { best_ms = per; best = v; }
       }
-      ctx->tune[{prec, row, n[c.axis], m}] = {best, best_ms};
+      ctx->tune[{prec, row, len, m}] = {best, best_ms};
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -851,10 +861,18 @@ struct Grid {
 // Grid sets and the 2D multigrid transfer (multigrid.h:617-657, mpi_logic.h:103-263)
 // ----------------------------------------------------------------------------------------
 struct Source { int region; int fc; };
+struct Prop;
 
 struct Set {
   qsf_ctx* ctx;
   std::vector<std::unique_ptr<Grid>> g;
+  // propagators created on this set; the last one created drives its grids (entry points that
+  // read or change a grid settle it first). Owned by their callers; detached when the set goes.
+  std::vector<Prop*> props;
+  Prop* active = nullptr;
+  ~Set();
+  Set(const Set&) = delete;
+  Set& operator=(const Set&) = delete;
   std::vector<std::vector<Source>> sources;
   int D, groupCount = 1;
   bool multi;
@@ -1021,13 +1039,9 @@ struct PendingX {
   XFactor f{};
 };
 
-// The propagator driving each set's grids (the last one created on it); entry points that
-// read or change a grid settle it first.
-struct Prop;
-static std::map<const Set*, Prop*> g_active;
-
 struct Prop {
-  Set* set;
+  Set* set;            // nullptr once the set was destroyed (the propagator is then inert)
+  qsf_ctx* ctx;        // outlives every set and propagator created on it
   qsf_prop_desc pd;
   phys::Coupling coup;
   std::vector<Chain> chains;
@@ -1051,8 +1065,8 @@ struct Prop {
   int pend_row = -1;
   bool pend_ekin = false;
 
-  Prop(Set* s, const qsf_prop_desc& p, const qsf_field_desc* f, int nf) : set(s), pd(p), coup(p.gauge, f, nf) {
-    g_active[s] = this;
+  Prop(Set* s, const qsf_prop_desc& p, const qsf_field_desc* f, int nf)
+      : set(s), ctx(s->ctx), pd(p), coup(p.gauge, f, nf) {
     QSF_REQUIRE(p.order >= 1 && p.order <= 8, QSF_ERR_ARG, "order must be 1..8");
     QSF_REQUIRE(p.nborders >= 0 && p.nborders <= 16, QSF_ERR_ARG, "0..16 borders");
     dt = p.dt;
@@ -1072,10 +1086,12 @@ struct Prop {
     // observables that change the state or steer the loop every step run unfused
     if (p.mode == QSF_MODE_IM || (p.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF))) exec = 1;
     if (p.nborders > 0) main().build_gather(p.borders, p.nborders);
+    s->props.push_back(this);
+    s->active = this;
   }
 
   Grid& main() { return *set->g[0]; }
-  hipStream_t st() { return set->ctx->stream; }
+  hipStream_t st() { return ctx->stream; }
   int couples() const { return coup.rep(); }
 
   // ---- host side of makeStep: chain bookkeeping, precalc side effects, substep list ----
@@ -1186,6 +1202,11 @@ struct Prop {
 
   bool fused_multi() const { return set->multi && set->D == 2; }
 
+  // grids whose fused P substep runs as one forward-factor-inverse pass per axis (3D main
+  // grids); 2D grids keep the row pass [inverse, X, forward] + column pass [forward, P,
+  // inverse], which is two sweeps either way and carries the multigrid transfer bookkeeping
+  static bool per_axis_kinetic(const Grid& g) { return g.D == 3 && g.fftAxes.size() == 3; }
+
   // multigrid post-compute work of the main region folded into its pass
   void add_post(Grid& g, PassParams& p) {
     set->ensure_edges();
@@ -1262,6 +1283,30 @@ struct Prop {
       pend[r].on = false;
       return;
     }
+    if (per_axis_kinetic(g)) {
+      // 3D main grid: the kinetic factor is a product of commuting one-axis factors,
+      // exp(-iT dt) = prod_d F_d^-1 f_d(p_d) F_d, so each axis pass transforms forward, applies
+      // its own axis' factor and transforms back. Three sweeps per P substep (the last-axis pass
+      // also carries the X halves and the observation); the state stays in X between passes.
+      for (int a = D - 1; a >= 0; --a) {
+        PassParams p = g.params(a, g.psi.p, g.psi.p);
+        p.flags = F_FWD | F_P | F_INV2;
+        p.pf[a] = pf[a];
+        if (a == D - 1) {
+          p.flags |= F_XB;
+          p.xb = fb;
+          if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
+          if (observe) obs_into(g, p);
+          int tiles = g.run(p, -1, observe ? obs_row() : nullptr);
+          if (observe) after_obs(g, tiles);
+        } else {
+          g.run(p);
+        }
+      }
+      deferred[r] = true;
+      pend[r].on = false;
+      return;
+    }
     // main grid, D >= 2: row pass on the last axis
     {
       PassParams p = g.params(D - 1, g.psi.p, g.psi.p);
@@ -1310,7 +1355,7 @@ struct Prop {
       const auto& ax = g.fftAxes;
       PassParams p = g.params(ax.size() == 1 ? ax[0] : g.D - 1, g.psi.p, g.psi.p);
       p.flags = 0;
-      if (ax.size() >= 2) p.flags |= F_INV1;
+      if (ax.size() >= 2 && !per_axis_kinetic(g)) p.flags |= F_INV1;   // pending inverse of the row axis
       if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
       if (r == 0 && post) add_post(g, p);
       if (observe) obs_into(g, p);
@@ -1409,13 +1454,12 @@ struct Prop {
 
   void flush_obs_only() { flush(); }
 
+
   bool fused() const { return exec == 0; }
   ~Prop() {
-    for (auto it = g_active.begin(); it != g_active.end(); ++it)
-      if (it->second == this) {
-        g_active.erase(it);
-        break;
-      }
+    if (!set) return;
+    set->props.erase(std::remove(set->props.begin(), set->props.end(), this), set->props.end());
+    if (set->active == this) set->active = nullptr;
   }
   Prop(const Prop&) = delete;
   Prop& operator=(const Prop&) = delete;
@@ -1460,6 +1504,10 @@ struct Prop {
     post_compute();
   }
 };
+
+Set::~Set() {
+  for (Prop* p : props) p->set = nullptr;
+}
 
 // ----------------------------------------------------------------------------------------
 // Batches: many single-region 2D solves (Strang, velocity gauge) advanced together. Solves are
@@ -1727,8 +1775,14 @@ struct Batch {
 
 using namespace qsf;
 
-struct qsf_set { Set s; qsf_set(qsf_ctx* c, const qsf_grid_desc& d) : s(c, d) {} };
 struct qsf_grid { Grid* g; Set* set = nullptr; };
+struct qsf_set {
+  Set s;
+  std::vector<std::unique_ptr<qsf_grid>> handles;   // one per region grid, owned by the set
+  qsf_set(qsf_ctx* c, const qsf_grid_desc& d) : s(c, d) {
+    for (auto& g : s.g) handles.emplace_back(new qsf_grid{g.get(), &s});
+  }
+};
 struct qsf_prop { Prop p; qsf_prop(Set* s, const qsf_prop_desc& d, const qsf_field_desc* f, int nf) : p(s, d, f, nf) {} };
 
 struct qsf_batch { Batch b; qsf_batch(qsf_ctx* c, const qsf_grid_desc* d, const qsf_field_desc* f, int n,
@@ -1749,20 +1803,8 @@ template <typename F> int guard(qsf_ctx* ctx, F&& f) {
     return QSF_ERR_ARG;
   }
 }
-std::map<const qsf_grid*, std::unique_ptr<qsf_grid>> g_handles;
-qsf_grid* handle_of(Grid* g, Set* s) {
-  for (auto& kv : g_handles)
-    if (kv.second->g == g) return kv.second.get();
-  auto h = std::make_unique<qsf_grid>();
-  h->g = g;
-  h->set = s;
-  qsf_grid* p = h.get();
-  g_handles[p] = std::move(h);
-  return p;
-}
 void settle(const Set* s) {
-  auto it = g_active.find(s);
-  if (it != g_active.end()) it->second->settle();
+  if (s->active) s->active->settle();
 }
 void settle(qsf_grid* g) {
   if (g->set) settle(g->set);
@@ -1922,13 +1964,7 @@ int qsf_set_create(qsf_ctx* c, const qsf_grid_desc* d, qsf_set** out) {
 int qsf_set_destroy(qsf_set* s) {
   if (!s) return QSF_ERR_ARG;
   (void)hipStreamSynchronize(s->s.ctx->stream);
-  g_active.erase(&s->s);
-  for (auto& g : s->s.g) {
-    for (auto it = g_handles.begin(); it != g_handles.end();)
-      if (it->second->g == g.get()) it = g_handles.erase(it);
-      else ++it;
-  }
-  delete s;
+  delete s;   // detaches its propagators; their own destroy stays valid
   return QSF_OK;
 }
 
@@ -1940,7 +1976,7 @@ int qsf_set_count(const qsf_set* s, int* n) {
 
 int qsf_set_grid(qsf_set* s, int r, qsf_grid** out) {
   if (!s || !out || r < 0 || r >= (int)s->s.g.size()) return QSF_ERR_ARG;
-  *out = handle_of(s->s.g[r].get(), &s->s);
+  *out = s->handles[r].get();
   return QSF_OK;
 }
 
@@ -2102,6 +2138,13 @@ int qsf_prop_destroy(qsf_prop* p) {
   delete p;
   return QSF_OK;
 }
+
+namespace {
+// a propagator whose set was destroyed first only accepts qsf_prop_destroy and the scalar queries
+void require_set(const qsf_prop* p) {
+  QSF_REQUIRE(p->p.set, QSF_ERR_STATE, "the propagator's grid set was destroyed");
+}
+}  // namespace
 int qsf_prop_max_steps(const qsf_prop* p, int64_t* o) {
   if (!p || !o) return QSF_ERR_ARG;
   *o = p->p.max_steps;
@@ -2109,11 +2152,13 @@ int qsf_prop_max_steps(const qsf_prop* p, int64_t* o) {
 }
 int qsf_prop_start(qsf_prop* p) {
   if (!p) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] { p->p.start(); });
+  return guard(p->p.ctx, [&] {
+    require_set(p); p->p.start(); });
 }
 int qsf_prop_fast_forward(qsf_prop* p, int64_t at_step) {
   if (!p || at_step < 0) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     QSF_REQUIRE(p->p.step == 0 && p->p.rows.empty(), QSF_ERR_STATE,
                 "fast_forward replaces start: call it on a fresh propagator");
     while (p->p.step < at_step) p->p.fake_step();
@@ -2126,11 +2171,13 @@ int qsf_prop_should_continue(qsf_prop* p, int* o) {
 }
 int qsf_prop_step(qsf_prop* p) {
   if (!p) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] { p->p.loop_body(); });
+  return guard(p->p.ctx, [&] {
+    require_set(p); p->p.loop_body(); });
 }
 int qsf_prop_run_steps(qsf_prop* p, int64_t k, int64_t* done) {
   if (!p) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     int64_t i = 0;
     while (i < k && p->p.should_continue()) {
       p->p.loop_body();
@@ -2141,7 +2188,8 @@ int qsf_prop_run_steps(qsf_prop* p, int64_t k, int64_t* done) {
 }
 int qsf_prop_finish(qsf_prop* p) {
   if (!p) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     p->p.loop_body();
     p->p.flush();
     p->p.sync_rows();
@@ -2169,7 +2217,8 @@ int qsf_prop_record_width(const qsf_prop* p, int* o) {
 }
 int qsf_prop_nrecords(qsf_prop* p, int* o) {
   if (!p || !o) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     p->p.flush();
     p->p.sync_rows();
     *o = (int)(p->p.rows.size() / p->p.width);
@@ -2177,7 +2226,8 @@ int qsf_prop_nrecords(qsf_prop* p, int* o) {
 }
 int qsf_prop_records(qsf_prop* p, double* out, int max_rows) {
   if (!p || !out) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     p->p.flush();
     p->p.sync_rows();
     const size_t n = std::min<size_t>((size_t)max_rows * p->p.width, p->p.rows.size());
@@ -2186,7 +2236,8 @@ int qsf_prop_records(qsf_prop* p, double* out, int max_rows) {
 }
 int qsf_prop_flux_integral(qsf_prop* p, double* out) {
   if (!p || !out) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     p->p.flush();
     QSF_HIP(hipMemcpyAsync(out, p->p.fluxint.p, p->p.pd.nborders * sizeof(double), hipMemcpyDeviceToHost,
                            p->p.st()));
@@ -2266,7 +2317,8 @@ int qsf_batch_flux_integral(qsf_batch* b, int s, double* out) {
 
 int qsf_prop_set_exec(qsf_prop* p, int mode) {
   if (!p || (mode != 0 && mode != 1)) return QSF_ERR_ARG;
-  return guard(p->p.set->ctx, [&] {
+  return guard(p->p.ctx, [&] {
+    require_set(p);
     QSF_REQUIRE(mode == 1 || !(p->p.pd.mode == QSF_MODE_IM || (p->p.pd.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF))),
                 QSF_ERR_UNSUPPORTED, "imaginary time, Normalize and ENERGY_DIFFERENCE run unfused");
     p->p.flush();

@@ -142,12 +142,12 @@ static int launch_tr(const TransferParams& p, hipStream_t s) {
   using G = Geo<Real, N, ROW, 0>;
   int64_t tiles = ROW ? (p.n0 + G::W - 1) / G::W : (p.n1 + G::W - 1) / G::W;
   size_t lds = (size_t)G::lds_cap(G::W);
-  static bool attr_set = false;
-  if (!attr_set) {
+  static const bool attr_set = [&] {   // once per kernel, also under concurrent callers
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&transfer_kernel<Real, N, ROW>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    attr_set = true;
-  }
+    return true;
+  }();
+  (void)attr_set;
   hipLaunchKernelGGL((transfer_kernel<Real, N, ROW>), dim3((unsigned)tiles), dim3(G::THREADS), lds, s, p);
   return (int)tiles;
 }

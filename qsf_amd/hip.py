@@ -371,10 +371,9 @@ class Propagator:
             self.ctx.check(load().qsf_prop_set_exec(h, exec_mode), "prop_set_exec")
 
     def close(self):
-        # the C propagator reaches its set's stream: once the set is gone (a garbage cycle can
-        # finalize the set first, after its weak list of propagators was cleared) it is left
-        if (getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None)
-                and getattr(self.gset, "h", None)):
+        # valid before or after the set's own destroy (the library detaches a propagator whose
+        # set went first); only the context must still be alive
+        if getattr(self, "h", None) and not _EXITING[0] and getattr(self.ctx, "h", None):
             load().qsf_prop_destroy(self.h)
         self.h = None
 

@@ -13,10 +13,30 @@ def crand(m, seed):
     return r.normal(size=m) + 1j * r.normal(size=m)
 
 
-@pytest.mark.parametrize("dim,particles,n", [(2, 2, 7), (3, 2, 5), (3, 3, 6), (1, 2, 9), (2, 3, 4)])
+@pytest.mark.parametrize("dim,particles,n", [(2, 2, 7), (3, 2, 5), (3, 3, 6), (1, 2, 9), (2, 3, 4),
+                                             (3, 2, (4, 6, 6)), (3, 2, (7, 3, 3))])
 def test_oracle_equals_reference_loops(dim, particles, n):
-    psi = crand(n ** dim, 1)
+    m = int(np.prod(n)) if isinstance(n, tuple) else n ** dim
+    psi = crand(m, 1)
     assert np.array_equal(F.antisymmetrize(psi, n, dim, particles), F.antisymmetrize_loops(psi, n, dim, particles))
+
+
+def test_oracle_fp32_order_is_float32_arithmetic():
+    # the complex64 form adds in float32, in the kernel's order: (b + c) - d - e - f, then a + t;
+    # check one node against explicit float32 scalar arithmetic
+    n = 5
+    psi = crand(n ** 3, 11).astype(np.complex64)
+    got = F.antisymmetrize(psi, n, 3, 3, dtype=np.complex64).reshape(n, n, n)
+    a = psi.reshape(n, n, n)
+    i, j, k = 4, 2, 1
+    f32 = np.float32
+    for part in ("real", "imag"):
+        g = lambda *ix: f32(getattr(a[ix], part))
+        t = f32(g(j, k, i) + g(k, i, j))
+        t = f32(t - g(j, i, k))
+        t = f32(t - g(i, k, j))
+        t = f32(t - g(k, j, i))
+        assert f32(g(i, j, k) + t) == getattr(got[i, j, k], part)
 
 
 def test_antisymmetry_and_projection():
@@ -36,17 +56,26 @@ def test_antisymmetry_and_projection():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dim,particles,n", [(2, 2, 64), (2, 2, 48), (3, 2, 16), (3, 3, 16), (3, 3, 12), (1, 2, 64)])
-def test_device_antisymmetrize_bit_exact(dim, particles, n):
+@pytest.mark.parametrize("dim,particles,n", [(2, 2, 64), (2, 2, 48), (3, 2, 16), (3, 3, 16), (3, 3, 12), (1, 2, 64),
+                                             (3, 2, (8, 16, 16)), (3, 2, (24, 12, 12))])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_device_antisymmetrize_bit_exact(dim, particles, n, prec):
     from qsf_amd import hip
     from qsf_amd import qsf_types as T
-    print(f"[case] antisymmetrize dim={dim} particles={particles} n={n}", flush=True)
+    print(f"[case] antisymmetrize dim={dim} particles={particles} n={n} prec={prec}", flush=True)
+    shape = list(n) if isinstance(n, tuple) else [n] * dim
+    precision = T.FP64 if prec == 0 else T.FP32_MIXED
     ctx = hip.Context(0)
-    gs = hip.GridSet(ctx, T.grid_desc(dim, n, 0.3))
-    psi = crand(n ** dim, 5)
+    gs = hip.GridSet(ctx, T.grid_desc(dim, shape, 0.3, precision=precision))
+    psi = crand(int(np.prod(shape)), 5)
+    if precision == T.FP64:
+        want = F.antisymmetrize(psi, n, dim, particles)
+    else:   # the FP32 state: float32 additions in the kernel's order
+        psi = psi.astype(np.complex64).astype(np.complex128)
+        want = F.antisymmetrize(psi, n, dim, particles, dtype=np.complex64).astype(np.complex128)
     gs[0].set_psi(psi)
     gs[0].antisymmetrize(particles)
-    assert np.array_equal(gs[0].psi(), F.antisymmetrize(psi, n, dim, particles))
+    assert np.array_equal(gs[0].psi(), want)
     gs.close()
     ctx.close()
 
