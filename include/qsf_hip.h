@@ -11,7 +11,7 @@
  * reference's row-major layout (X rep) or FFTW's transposed P layout (D>1 main grids,
  * wf/grid.h:238-248). One HIP stream per context; calls on one context are not thread-safe,
  * different contexts may be used from different threads (the library keeps no process-wide
- * mutable state).
+ * mutable state apart from the per-device scratch of qsf_average_noalloc).
  */
 #ifndef QSF_HIP_H
 #define QSF_HIP_H
@@ -89,6 +89,16 @@ int qsf_fourier(qsf_grid* g, int to_rep);
 int qsf_evolve(qsf_grid* g, int mode, int rep, double delta, const double* coupling3, int couples_in);
 /* average<REP,Op>() grid.h:425-441; op 0 = Identity, 1 = PotentialEnergy, 2 = KineticEnergy. */
 int qsf_average(qsf_grid* g, int rep, int op, double* out);
+/* qsf_average that allocates nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The observation pass's block sums and their reduction go to a static
+ * per-device scratch array of 8 MiB (held by each device until the process ends), so grids whose
+ * last-axis pass has more than (8 MiB / 8 - 776) / 3 = 349,266 blocks are refused (QSF_ERR_ARG;
+ * every grid up to 512^3 fits). The grid must be at a step boundary: QSF_ERR_STATE while a
+ * propagator step is pending on its set (any other grid entry point or qsf_prop_finish settles
+ * it). All calls on one device share the scratch: calls that overlap on one device (two
+ * contexts, or two threads) race. Same kernels and reduction order as qsf_average. */
+int qsf_average_noalloc(qsf_grid* g, int rep, int op, double* out);
 /* operation<M,R,Normalize>() grid.h:599-606. */
 int qsf_normalize(qsf_grid* g);
 /* current_map() + velocityGaugeCorrection + flux<R,F>() (grid.h:467-573, propagator.h:249-265).

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -78,6 +79,7 @@ struct DBuf {
 struct qsf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  double* scratch = nullptr;   // this device's address of the no-allocation scratch array
   std::string err;
   hipEvent_t ev[2] = {nullptr, nullptr};
   int marks = 0;
@@ -113,7 +115,35 @@ struct qsf_ctx {
   }
 };
 
+// ---- scratch of qsf_average_noalloc ----
+// One array per device, held until the process ends: [0, 8) the three sums of the result,
+// [8, 8 + 256*3) the second reduction level, then 3 partial sums per block of the observation
+// pass. Device code may name it; host code uses only the per-device address from
+// hipGetSymbolAddress (the host-side name is a stand-in, not a device address).
 namespace qsf {
+constexpr size_t kScratchBytes = size_t(8) << 20;
+constexpr int64_t kScratchResult = 0, kScratchLevel = 8, kScratchPartials = 8 + 256 * 3;
+constexpr int64_t kScratchMaxTiles = ((int64_t)(kScratchBytes / sizeof(double)) - kScratchPartials) / 3;
+constexpr int kMaxDevices = 64;
+}  // namespace qsf
+static __device__ __align__(256) double scratch[qsf::kScratchBytes / sizeof(double)];
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[qsf::kMaxDevices] = {};
+
+namespace qsf {
+
+// The address of `scratch` on the current device (taken once per device, with it current).
+inline double* scratch_address(int device) {
+  QSF_REQUIRE(device >= 0 && device < kMaxDevices, QSF_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    QSF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QSF_REQUIRE(p, QSF_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_addr[device] = static_cast<double*>(p);
+  }
+  return g_scratch_addr[device];
+}
 
 // Fold recorded launch events into per-kernel totals (synchronises the stream).
 inline void drain_prof(qsf_ctx* c) {
@@ -446,7 +476,7 @@ struct Grid {
     for (int a = 0; a < 3; ++a) pf[a] = a < D ? args[a].out : nullptr;
   }
 
-  PassParams params(int axis, const void* src, void* dst) {
+  PassParams params(int axis, const void* src, void* dst, bool with_roots = true) {
     PassParams p{};
     p.src = src;
     p.dst = dst;
@@ -456,7 +486,7 @@ struct Grid {
     for (int i = axis + 1; i < 3; ++i) p.inner *= n[i];
     p.outer = 1;
     for (int i = 0; i < axis; ++i) p.outer *= n[i];
-    p.roots = root(n[axis]);
+    p.roots = with_roots ? root(n[axis]) : nullptr;   // root() builds the table on first use
     p.scale = 1.0;
     p.nCAP = d.nCAP;
     return p;
@@ -631,6 +661,35 @@ struct Grid {
     observe(op == 1, op == 2, r);
     const double vol = R == QSF_REP_X ? dV : dVm;  // vol<R> coords.h:48-53
     return r[op] * vol;
+  }
+
+  // average<R, op> with the block partials and the reduction in `scr` (the per-device scratch):
+  // the same observation pass (same kernel variant, so the same blocks) and the same fixed-order
+  // reduction as average(), with no table build and no buffer of the grid touched or grown.
+  double average_into(int R, int op, double* scr) {
+    PassParams p = params(D - 1, psi.p, psi.p, false);
+    p.flags = F_OBS | F_NOSTORE;
+    p.tw_mode = ctx->tw_mode[prec];
+    obs_params(p.obs, op == 1);
+    if (op == 2) {
+      int64_t maxn = *std::max_element(n, n + 3);
+      for (int a = 0; a < D; ++a) p.obs.kin_w[a] = kinw.as<double>() + a * maxn;
+    }
+    const int var = ctx->variant_of(prec, p.inner == 1, n[p.axis], m);
+    const int64_t tiles = axis_pass_tiles(prec, p, var);
+    QSF_REQUIRE(tiles >= 1 && tiles <= kScratchMaxTiles, QSF_ERR_ARG,
+                "the observation pass of this grid has " + std::to_string(tiles) + " blocks; the "
+                "no-allocation scratch holds at most " + std::to_string(kScratchMaxTiles));
+    double* res = scr + kScratchResult;
+    p.obs.partials = scr + kScratchPartials;
+    QSF_REQUIRE(launch_axis_pass(prec, p, st(), var) >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
+    QSF_HIP(hipGetLastError());
+    launch_reduce_partials(p.obs.partials, tiles, 3, res, scr + kScratchLevel, st());
+    QSF_HIP(hipGetLastError());
+    double r[3];
+    QSF_HIP(hipMemcpyAsync(r, res, 3 * sizeof(double), hipMemcpyDeviceToHost, st()));
+    QSF_HIP(hipStreamSynchronize(st()));
+    return r[op] * (R == QSF_REP_X ? dV : dVm);   // vol<R> coords.h:48-53
   }
 
   void normalize() {
@@ -1454,6 +1513,13 @@ struct Prop {
 
   void flush_obs_only() { flush(); }
 
+  // the fused pipeline left work for the next pass (the grids are not at the step boundary)
+  bool pending() const {
+    if (!fused()) return false;
+    for (size_t r = 0; r < deferred.size(); ++r)
+      if (deferred[r] || pend[r].on) return true;
+    return pend_obs || pend_post;
+  }
 
   bool fused() const { return exec == 0; }
   ~Prop() {
@@ -1825,6 +1891,7 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
     QSF_HIP(hipGetDeviceCount(&count));
     QSF_REQUIRE(device >= 0 && device < count, QSF_ERR_ARG, "no such device");
     QSF_HIP(hipSetDevice(device));
+    c->scratch = scratch_address(device);
     QSF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     QSF_HIP(hipEventCreate(&c->ev[0]));
     QSF_HIP(hipEventCreate(&c->ev[1]));
@@ -2035,6 +2102,20 @@ int qsf_average(qsf_grid* g, int rep, int op, double* out) {
     QSF_REQUIRE(rep == g->g->rep, QSF_ERR_STATE, "average: grid is in the other representation");
     QSF_REQUIRE(op >= 0 && op <= 2, QSF_ERR_ARG, "op must be 0..2");
     *out = g->g->average(rep, op);
+  });
+}
+int qsf_average_noalloc(qsf_grid* g, int rep, int op, double* out) {
+  if (!g || !out) return QSF_ERR_ARG;
+  return guard(g->g->ctx, [&] {
+    Grid& G = *g->g;
+    QSF_REQUIRE(G.ctx->scratch, QSF_ERR_STATE, "no scratch address on this context's device");
+    // settling would run the pending passes through the grid's own buffers: refused instead
+    QSF_REQUIRE(!(g->set && g->set->active && g->set->active->pending()), QSF_ERR_STATE,
+                "a propagator step is pending on this grid set; settle it first (any other grid entry "
+                "point, or qsf_prop_finish)");
+    QSF_REQUIRE(rep == G.rep, QSF_ERR_STATE, "average: grid is in the other representation");
+    QSF_REQUIRE(op >= 0 && op <= 2, QSF_ERR_ARG, "op must be 0..2");
+    *out = G.average_into(rep, op, G.ctx->scratch);
   });
 }
 int qsf_normalize(qsf_grid* g) {

@@ -43,6 +43,7 @@ SIGNATURES = {
     "qsf_fourier": (_I, [_P, _I]),
     "qsf_evolve": (_I, [_P, _I, _I, _D, _dp, _I]),
     "qsf_average": (_I, [_P, _I, _I, _dp]),
+    "qsf_average_noalloc": (_I, [_P, _I, _I, _dp]),
     "qsf_normalize": (_I, [_P]),
     "qsf_flux": (_I, [_P, _dp, C.POINTER(T.BorderDesc), _I, _dp]),
     "qsf_mask_region": (_I, [_P]),
@@ -286,6 +287,12 @@ class Grid:
     def average(self, rep, op):
         v = C.c_double()
         self.ctx.check(load().qsf_average(self.h, rep, op, C.byref(v)), "average")
+        return v.value
+
+    def average_noalloc(self, rep, op):
+        """average() through the entry point that allocates no device memory."""
+        v = C.c_double()
+        self.ctx.check(load().qsf_average_noalloc(self.h, rep, op, C.byref(v)), "average_noalloc")
         return v.value
 
     def normalize(self):
