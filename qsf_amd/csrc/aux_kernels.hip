@@ -132,13 +132,13 @@ __global__ void reduce_partials_kernel(const double* partials, int64_t count, in
     double acc = 0.0;
     for (int64_t k = threadIdx.x; k < count; k += 256) acc += partials[k * width + c];
     red[threadIdx.x] = acc;
-    __syncthreads();
+    block_sync();
     for (int s = 128; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+      block_sync();
     }
     if (threadIdx.x == 0) out[c] = red[0];
-    __syncthreads();
+    block_sync();
   }
 }
 
@@ -150,13 +150,13 @@ __global__ void reduce_chunks_kernel(const double* partials, int64_t count, int 
     double acc = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += 256) acc += partials[k * width + c];
     red[threadIdx.x] = acc;
-    __syncthreads();
+    block_sync();
     for (int s = 128; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+      block_sync();
     }
     if (threadIdx.x == 0) out[blockIdx.x * width + c] = red[0];
-    __syncthreads();
+    block_sync();
   }
 }
 
@@ -204,10 +204,10 @@ __global__ void flux_kernel(FluxArgs a) {
   __shared__ double red[256];
   for (int b = 0; b < nb; ++b) {
     red[threadIdx.x] = acc[b];
-    __syncthreads();
+    block_sync();
     for (int s = 128; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+      block_sync();
     }
     if (threadIdx.x == 0) {
       if (gridDim.x == 1) {
@@ -219,7 +219,7 @@ __global__ void flux_kernel(FluxArgs a) {
         a.partials[blockIdx.x * nb + b] = red[0];
       }
     }
-    __syncthreads();
+    block_sync();
   }
 }
 
@@ -231,16 +231,16 @@ __global__ void flux_finish_kernel(const double* partials, int64_t count, int nb
     double acc = 0.0;
     for (int64_t k = threadIdx.x; k < count; k += blockDim.x) acc += partials[k * nb + b];
     red[threadIdx.x] = acc;
-    __syncthreads();
+    block_sync();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+      block_sync();
     }
     if (threadIdx.x == 0) {
       out[b] = red[0] * scale;
       if (accum) accum[b] += out[b] * w;
     }
-    __syncthreads();
+    block_sync();
   }
 }
 
@@ -483,13 +483,13 @@ __global__ void overlap_partials_kernel(const double2* phi, const T* psi, int64_
   }
   sr[threadIdx.x] = ar;
   si[threadIdx.x] = ai;
-  __syncthreads();
+  block_sync();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       sr[threadIdx.x] += sr[threadIdx.x + w];
       si[threadIdx.x] += si[threadIdx.x + w];
     }
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) {
     partials[2 * blockIdx.x] = sr[0];
@@ -588,13 +588,13 @@ __global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, con
   double acc = 0.0;
   for (int64_t b = threadIdx.x; b < a.tiles; b += 256) acc += a.partials[3 * b];
   red[threadIdx.x] = acc;
-  __syncthreads();
+  block_sync();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) row[0] = red[0] * a.dV;
-  __syncthreads();
+  block_sync();
   const double* C = a.coupling + 3 * (k - 1);   // A after the step's P substep
   const T* g = reinterpret_cast<const T*>(a.gathered);
   const int D = geo.D, nb = geo.nb, stride = 1 + 2 * D;
@@ -620,16 +620,16 @@ __global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, con
       s += j;
     }
     red[threadIdx.x] = s;
-    __syncthreads();
+    block_sync();
     for (int t = 128; t > 0; t >>= 1) {
       if ((int)threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
-      __syncthreads();
+      block_sync();
     }
     if (threadIdx.x == 0) {
       row[1 + f] = red[0] * geo.scale;
       a.flux_int[f] += row[1 + f] * geo.dt;
     }
-    __syncthreads();
+    block_sync();
   }
 }
 

@@ -10,6 +10,7 @@
 // with the X halves of consecutive substeps meeting inside one pass.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -40,6 +41,16 @@ template <typename T, int N> constexpr int row_stride() { return N + (N >> RowAd
 constexpr int kVariants = 6;
 constexpr int var_alt(int v) { return v / 3; }
 constexpr int var_wshift(int v) { return v % 3 == 0 ? 0 : (v % 3 == 1 ? -1 : 1); }
+
+// Compile-time feature set of a kernel. A kernel's register allocation covers every path it
+// contains, so passes that only transform and multiply by the P factor (the column passes, the
+// middle passes, plain transforms) run a build without the X stage, the observation, the flux
+// gather, the region mask and the multigrid edge writes: far fewer registers, more waves.
+enum PassFeat : int { FEAT_FULL = 0, FEAT_LEAN = 1 };
+constexpr uint32_t kLeanFlags = F_INV1 | F_FWD | F_P | F_INV2 | F_SCALE | F_NOSTORE;
+inline int feat_of(const PassParams& p) {
+  return ((p.flags & ~kLeanFlags) == 0 && !p.er_out) ? FEAT_LEAN : FEAT_FULL;
+}
 
 // Block geometry per (precision, N, mode, variant).
 template <typename Real, int N, bool ROW, int V = 0> struct Geo {
@@ -229,100 +240,96 @@ __device__ __forceinline__ double pot_value(const ObsParams& o, const double* c,
   return v;
 }
 
-// QSF_MIN_WAVES: minimum waves per SIMD the register allocation must allow.
-#ifndef QSF_MIN_WAVES
-#define QSF_MIN_WAVES 1
-#endif
-template <typename Real, int N, bool ROW, int V>
-__global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES)
-    axis_pass_kernel(PassParams p) {
+// Geometry of one thread's transform in tile b: (outer o, inner c), its global offsets and the
+// grid coordinates of its line (the transform axis index is per element).
+template <int N, bool ROW, int W>
+struct TileGeo {
+  int64_t o, c, base, step;
+  bool valid;
+  int i0, i1, i2;
+  __device__ __forceinline__ void at(const PassParams& p, int64_t b, int w) {
+    if (ROW) {
+      o = b * W + w;
+      c = 0;
+      valid = o < p.outer;
+    } else {
+      const int64_t cpt = (p.inner + W - 1) / W;  // column tiles per outer index
+      o = b / cpt;
+      c = (b % cpt) * W + w;
+      valid = (o < p.outer) && (c < p.inner);
+    }
+    base = ROW ? o * N : o * (int64_t)N * p.inner + c;
+    step = ROW ? 1 : p.inner;
+    i0 = i1 = i2 = 0;
+    if (p.axis == 0) {
+      i1 = (int)(c / p.n[2]);
+      i2 = (int)(c % p.n[2]);
+    } else if (p.axis == 1) {
+      i0 = (int)o;
+      i2 = (int)c;
+    } else {
+      i0 = (int)(o / p.n[1]);
+      i1 = (int)(o % p.n[1]);
+    }
+    if (!valid) i0 = i1 = i2 = 0;   // idle threads of a partial tile read in-range table entries
+  }
+};
+
+template <int W, bool ROW>
+__device__ __forceinline__ int64_t tile_count(const PassParams& p) {
+  return ROW ? (p.outer + W - 1) / W : p.outer * ((p.inner + W - 1) / W);
+}
+
+// Tile of the k-th position in launch order (k in [0, ntiles)). 3D rows (axis 2) read pair-table
+// rows indexed by i1 and by i0 (the (1,2) and (2,0) pairs). Walking lines with i1 fastest cycles
+// all n rows of the table through L2 every n lines (512 x 8 KiB at 512^3: re-fetched from HBM,
+// +2 GiB per pass); walking (i0, i1) in 32 x 64 patches keeps the rows the in-flight tiles of
+// one XCD use resident.
+template <int W, bool ROW>
+__device__ __forceinline__ int64_t tile_at(const PassParams& p, int64_t k, int64_t ntiles) {
+  if (ROW && p.axis == 2 && p.n[2] > 1) {
+    constexpr int64_t P0 = 32, P1 = 64;
+    const int64_t n0 = p.n[0], n1 = p.n[1];
+    if (n0 % P0 == 0 && n1 % P1 == 0 && P1 % W == 0 && ntiles * W == n0 * n1) {
+      const int64_t tpp = P0 * (P1 / W);  // tiles per patch
+      const int64_t pidx = k / tpp, r = k % tpp, pc = n1 / P1;
+      const int64_t j0 = (pidx / pc) * P0 + r / (P1 / W);
+      const int64_t j1 = (pidx % pc) * P1 + (r % (P1 / W)) * W;
+      return (j0 * n1 + j1) / W;  // the tile whose first line is (j0, j1)
+    }
+  }
+  return k;
+}
+
+// One tile of the pass on the state `v` already loaded: everything after the load, through the
+// store. Block sums of an observation are added to acc[] by thread 0.
+template <typename Real, int N, bool ROW, int V, int FEAT>
+__device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, ROW, Geo<Real, N, ROW, V>::W>& g,
+                                          typename Tr<Real>::T (&v)[PlanOf<N, var_alt(V)>::E], int tid, int t, int w,
+                                          unsigned char* smem_raw, typename Tr<Real>::T* dst, const void* pf0,
+                                          const void* pf1, const void* pf2, void* gather_out, double* acc) {
   using T = typename Tr<Real>::T;
   constexpr int ALT = var_alt(V);
   using PL = PlanOf<N, ALT>;
   constexpr int E = PL::E, TT = PL::T;
   constexpr int W = Geo<Real, N, ROW, V>::W;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-
-  const int tid = threadIdx.x;
-  const int t = ROW ? (tid % TT) : (tid / W);
-  const int w = ROW ? (tid / TT) : (tid % W);
-
-  // XCD-aware tile order: consecutive tiles land on the same XCD (blocks go round-robin).
-  int64_t b = blockIdx.x;
-  const int64_t nb = gridDim.x;
-  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-  if (ROW && p.axis == 2 && p.n[2] > 1) {
-    // 3D rows (axis 2) read pair-table rows indexed by i1 and by i0 (the (1,2) and (2,0) pairs).
-    // Walking lines with i1 fastest cycles all n rows of the table through L2 every n lines
-    // (512 x 8 KiB at 512^3: re-fetched from HBM, +2 GiB per pass); walking (i0, i1) in 32 x 64
-    // patches keeps the rows the in-flight tiles of one XCD use resident.
-    constexpr int64_t P0 = 32, P1 = 64;
-    const int64_t n0 = p.n[0], n1 = p.n[1];
-    if (n0 % P0 == 0 && n1 % P1 == 0 && P1 % W == 0 && nb * W == n0 * n1) {
-      const int64_t tpp = P0 * (P1 / W);  // tiles per patch
-      const int64_t pidx = b / tpp, r = b % tpp, pc = n1 / P1;
-      const int64_t j0 = (pidx / pc) * P0 + r / (P1 / W);
-      const int64_t j1 = (pidx % pc) * P1 + (r % (P1 / W)) * W;
-      b = (j0 * n1 + j1) / W;  // the tile whose first line is (j0, j1)
-    }
-  }
-
-  // tile -> (outer o, inner c) of this thread's transform
-  int64_t o, c;
-  bool valid;
-  if (ROW) {
-    o = b * W + w;
-    c = 0;
-    valid = o < p.outer;
-  } else {
-    const int64_t cpt = (p.inner + W - 1) / W;  // column tiles per outer index
-    o = b / cpt;
-    c = (b % cpt) * W + w;
-    valid = (o < p.outer) && (c < p.inner);
-  }
-  const int64_t base = ROW ? o * N : o * (int64_t)N * p.inner + c;
-  const int64_t step = ROW ? 1 : p.inner;
-
-  // coordinates of this thread's transform (the transform axis index is per element)
-  int i0 = 0, i1 = 0, i2 = 0;
-  if (p.axis == 0) {
-    i1 = (int)(c / p.n[2]);
-    i2 = (int)(c % p.n[2]);
-  } else if (p.axis == 1) {
-    i0 = (int)o;
-    i2 = (int)c;
-  } else {
-    i0 = (int)(o / p.n[1]);
-    i1 = (int)(o % p.n[1]);
-  }
-  if (!valid) i0 = i1 = i2 = 0;   // idle threads of a partial tile read in-range table entries
+  const T* roots = reinterpret_cast<const T*>(p.roots);
+  const int64_t o = g.o, base = g.base, step = g.step;
+  const bool valid = g.valid;
+  const int i0 = g.i0, i1 = g.i1, i2 = g.i2;
   auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
 
-  const T* roots = reinterpret_cast<const T*>(p.roots);
-  const T* src = reinterpret_cast<const T*>(p.src);
-  T* dst = reinterpret_cast<T*>(p.dst);
-  const void* pf0 = p.pf[0];
-  const void* pf1 = p.pf[1];
-  const void* pf2 = p.pf[2];
-  double* partials = p.obs.partials;
-  void* gather_out = p.obs.gather_out;
-  if (p.batch) {
-    // batched launch: this block's solve (whole block leaves together, before any barrier)
-    const BatchEntry& be = p.batch[blockIdx.y];
-    if (*p.step_ctr >= be.total_steps) return;
-    src = reinterpret_cast<const T*>(be.psi);
-    dst = reinterpret_cast<T*>(be.psi);
-    pf0 = be.pf[0];
-    pf1 = be.pf[1];
-    pf2 = be.pf[2];
-    partials = be.partials;
-    gather_out = be.gather_out;
+  // this thread's twiddles for every transform of the pass, and the flux gather's row range,
+  // read with the state
+  Twiddles<N, ALT, T, FEAT == FEAT_LEAN> tw;
+  if (p.flags & (F_INV1 | F_FWD | F_INV2)) tw.load(roots, t);
+  int gk0 = 0, gk1 = 0;
+  if (FEAT == FEAT_FULL && ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && valid) {
+    gk0 = p.obs.gather_rowptr[o];
+    gk1 = p.obs.gather_rowptr[o + 1];
   }
 
-  T v[E];
-#pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = valid ? src[base + (int64_t)(t + TT * q) * step] : T{Real(0), Real(0)};
   if (p.flags & F_SCALE) {
     // the scale (1/m of an inverse transform: 1/768 has no exact FP32 form) is applied in FP64
     // with one rounding; a once-rounded FP32 scale would bias the norm the same way every step
@@ -334,10 +341,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
     constexpr int DIR = decltype(dir_tag)::value;
     if constexpr (ROW) {
       RowAddr<T> a{w * row_stride<T, N>()};
-      fft<N, ALT, DIR>(v, t, lds, a, roots, p.tw_mode);
+      fft<N, ALT, DIR>(v, t, lds, a, roots, tw);
     } else {
       ColAddr<T, W> a{w};
-      fft<N, ALT, DIR>(v, t, lds, a, roots, p.tw_mode);
+      fft<N, ALT, DIR>(v, t, lds, a, roots, tw);
     }
   };
   using FWD = std::integral_constant<int, -1>;
@@ -347,7 +354,7 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
 
   // X stage: A factor -> observables / region mask -> B factor, one loop over the elements.
   // When both halves use the same pair-interaction table its lookup is shared.
-  const uint32_t XF = p.flags & (F_XA | F_OBS | F_MASK | F_XB);
+  const uint32_t XF = FEAT == FEAT_FULL ? p.flags & (F_XA | F_OBS | F_MASK | F_XB) : 0u;
   if (XF) {
     const bool doA = p.flags & F_XA, doB = p.flags & F_XB, obs = p.flags & F_OBS, msk = p.flags & F_MASK;
     XEval<N> xa, xb;
@@ -462,25 +469,24 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
         ep += __shfl_xor(ep, off);
         ek += __shfl_xor(ek, off);
       }
-      __syncthreads();
+      block_sync();
       if (gather && valid) {
-        const int k0 = ob.gather_rowptr[o], k1 = ob.gather_rowptr[o + 1];
         T* go = reinterpret_cast<T*>(gather_out);
-        for (int k = k0 + t; k < k1; k += TT) go[k] = lds[w * N + ob.gather_pos[k]];
+        for (int k = gk0 + t; k < gk1; k += TT) go[k] = lds[w * N + ob.gather_pos[k]];
       }
-      __syncthreads();
+      block_sync();
       double* red = reinterpret_cast<double*>(smem_raw);
       const int wave = tid >> 6, nwaves = (blockDim.x + 63) >> 6;
       if ((tid & 63) == 0) { red[3 * wave] = nrm; red[3 * wave + 1] = ep; red[3 * wave + 2] = ek; }
-      __syncthreads();
-      if (tid == 0 && partials) {
+      block_sync();
+      if (tid == 0) {
         double a = 0.0, e2 = 0.0, e3 = 0.0;
         for (int k = 0; k < nwaves; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
-        partials[3 * blockIdx.x] = a;
-        partials[3 * blockIdx.x + 1] = e2;
-        partials[3 * blockIdx.x + 2] = e3;
+        acc[0] += a;
+        acc[1] += e2;
+        acc[2] += e3;
       }
-      __syncthreads();
+      block_sync();
     }
   }
 
@@ -508,24 +514,95 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS), QSF_MIN_WAVES
   }
 }
 
+// Waves per SIMD the register allocation must leave room for. Left alone, the scheduler issues
+// every table load of the unrolled element loop at once and a full kernel lands at 169-171
+// VGPRs: two waves per SIMD, where 168 would give three (measured: the 512^3 row pass 2.4 ms at
+// three waves, 3.6 ms at two). Plans with 8 elements per thread fit three waves (full) or four
+// (lean, with its twiddles preloaded; the column tiles' LDS allows no more) without spilling,
+// 16-element plans two (full) or three (lean); the 24-element plans of the 3*2^k lengths need
+// their registers and are left uncapped.
+template <int E, int FEAT>
+constexpr int pass_waves() {
+  if (E <= 8) return FEAT == FEAT_LEAN ? 4 : 3;
+  if (E <= 16) return FEAT == FEAT_LEAN ? 3 : 2;
+  return 1;
+}
 
+template <typename Real, int N, bool ROW, int V, int FEAT>
+__global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
+                                  (pass_waves<PlanOf<N, var_alt(V)>::E, FEAT>()))
+    axis_pass_kernel(PassParams p) {
+  using T = typename Tr<Real>::T;
+  constexpr int ALT = var_alt(V);
+  using PL = PlanOf<N, ALT>;
+  constexpr int E = PL::E, TT = PL::T;
+  constexpr int W = Geo<Real, N, ROW, V>::W;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+
+  const int tid = threadIdx.x;
+  const int t = ROW ? (tid % TT) : (tid / W);
+  const int w = ROW ? (tid / TT) : (tid % W);
+
+  const T* src = reinterpret_cast<const T*>(p.src);
+  T* dst = reinterpret_cast<T*>(p.dst);
+  const void* pf0 = p.pf[0];
+  const void* pf1 = p.pf[1];
+  const void* pf2 = p.pf[2];
+  double* partials = p.obs.partials;
+  void* gather_out = p.obs.gather_out;
+  if (p.batch) {
+    // batched launch: this block's solve (whole block leaves together, before any barrier)
+    const BatchEntry& be = p.batch[blockIdx.y];
+    if (*p.step_ctr >= be.total_steps) return;
+    src = reinterpret_cast<const T*>(be.psi);
+    dst = reinterpret_cast<T*>(be.psi);
+    pf0 = be.pf[0];
+    pf1 = be.pf[1];
+    pf2 = be.pf[2];
+    partials = be.partials;
+    gather_out = be.gather_out;
+  }
+  const int64_t ntiles = tile_count<W, ROW>(p);
+  const int64_t nb = gridDim.x;
+  // XCD-aware order: blocks go round-robin over the 8 XCDs, so block i takes position
+  // (i % 8) * (nb / 8) + i / 8 and each XCD walks a contiguous run of tiles (its L2 sees the
+  // neighbours' halves of shared lines)
+  const int64_t k0 = (nb & 7) == 0 ? (int64_t)(blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3) : (int64_t)blockIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  TileGeo<N, ROW, W> g;
+  g.at(p, tile_at<W, ROW>(p, k0, ntiles), w);
+  T v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = g.valid ? src[g.base + (int64_t)(t + TT * q) * g.step] : T{Real(0), Real(0)};
+  pass_tile<Real, N, ROW, V, FEAT>(p, g, v, tid, t, w, smem_raw, dst, pf0, pf1, pf2, gather_out, acc);
+  if (FEAT == FEAT_FULL && (p.flags & F_OBS) && tid == 0 && partials) {
+    partials[3 * blockIdx.x] = acc[0];
+    partials[3 * blockIdx.x + 1] = acc[1];
+    partials[3 * blockIdx.x + 2] = acc[2];
+  }
+}
+
+// Blocks of one launch: one per tile.
 template <typename Real, int N, bool ROW, int V>
-inline int launch_one(const PassParams& p, hipStream_t s) {
+inline int64_t launch_blocks(const PassParams& p) {
   using G = Geo<Real, N, ROW, V>;
-  int64_t tiles;
-  if (ROW) tiles = (p.outer + G::W - 1) / G::W;
-  else tiles = p.outer * ((p.inner + G::W - 1) / G::W);
+  return ROW ? (p.outer + G::W - 1) / G::W : p.outer * ((p.inner + G::W - 1) / G::W);
+}
+
+template <typename Real, int N, bool ROW, int V, int FEAT>
+inline int launch_feat(const PassParams& p, hipStream_t s) {
+  using G = Geo<Real, N, ROW, V>;
   // LDS per launch: the block reduction (3 sums per wave), the transform exchange, and for a
   // flux gather the complex lines themselves; a pointwise pass (a region whose axes all stay in
   // P) needs only the reduction, so more blocks fit on a CU
   const size_t red_bytes = 16 * 3 * sizeof(double);
   const size_t xchg_bytes = (size_t)G::lds_cap(G::W);
-  const size_t gather_bytes = (size_t)G::W * N * G::CB;
+  const size_t gather_bytes = FEAT == FEAT_FULL ? (size_t)G::W * N * G::CB : 0;
   // once per kernel (a function-local static initialises once, also under concurrent callers)
   static const bool attr_set = [&] {
     size_t most = red_bytes > xchg_bytes ? red_bytes : xchg_bytes;
     if (ROW && gather_bytes > most) most = gather_bytes;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V, FEAT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
     return true;
   }();
@@ -534,15 +611,22 @@ inline int launch_one(const PassParams& p, hipStream_t s) {
   if ((p.flags & (F_INV1 | F_FWD | F_INV2)) && xchg_bytes > lds) lds = xchg_bytes;
   if (ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && gather_bytes > lds) lds = gather_bytes;
   const unsigned ny = p.batch ? (unsigned)p.nbatch : 1u;
-  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V>), dim3((unsigned)tiles, ny), dim3(G::THREADS), lds, s, p);
-  return (int)tiles;
+  const int64_t blocks = launch_blocks<Real, N, ROW, V>(p);
+  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V, FEAT>), dim3((unsigned)blocks, ny), dim3(G::THREADS), lds, s,
+                     p);
+  return (int)blocks;
 }
 
 template <typename Real, int N, bool ROW, int V>
+inline int launch_one(const PassParams& p, hipStream_t s) {
+  return feat_of(p) == FEAT_LEAN ? launch_feat<Real, N, ROW, V, FEAT_LEAN>(p, s)
+                                 : launch_feat<Real, N, ROW, V, FEAT_FULL>(p, s);
+}
+
+// Blocks of a launch, which is also the number of per-block observation sums it writes.
+template <typename Real, int N, bool ROW, int V>
 inline int64_t tiles_one(const PassParams& p) {
-  using G = Geo<Real, N, ROW, V>;
-  if (ROW) return (p.outer + G::W - 1) / G::W;
-  return p.outer * ((p.inner + G::W - 1) / G::W);
+  return launch_blocks<Real, N, ROW, V>(p);
 }
 
 // Lengths with tuned variants; every other supported length has variant 0 only.

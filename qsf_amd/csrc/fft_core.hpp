@@ -14,6 +14,18 @@
 
 namespace qsf {
 
+// Block barrier for data exchanged through LDS. __syncthreads() is s_barrier between two
+// workgroup fences, and the compiler has been seen to sink an LDS read of an exchange below it
+// (into the branch of the next butterfly stage, which needs that element last); the read then
+// races with other waves' writes of the next exchange and the transform comes out wrong, at
+// random. An empty asm statement with a memory clobber on each side is a code-motion barrier for
+// every memory access and emits no instruction.
+__device__ __forceinline__ void block_sync() {
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  asm volatile("" ::: "memory");
+}
+
 template <typename R> struct cvec;
 template <> struct cvec<double> { using T = double2; };
 template <> struct cvec<float> { using T = float2; };
@@ -220,6 +232,54 @@ template <int N, int ALT = 0> struct PlanOf {
   }
 };
 
+// ---------------------------------------------------------------- per-thread twiddles
+// The twiddle roots a thread needs: for every stage after the first and every butterfly b of the
+// thread, w^k and (radix > 4) w^4k with k = (t + T b) mod NS; the other powers follow by FP64
+// products. When they fit in six 16-byte records (the 8- and 16-element plans) they are read once
+// when the pass starts, next to the state loads, so no stage waits on a table load behind its
+// exchange barrier; forward and inverse transforms of the pass share them (conjugated). Larger
+// sets (the 24-element plans of the 3*2^k lengths), and kernels that keep PRE = false because
+// their registers are spoken for, read them stage by stage.
+template <typename T> struct RootRec;
+template <> struct RootRec<double2> { using type = double2; };   // the FP64 root
+template <> struct RootRec<float2> { using type = float4; };     // FP32 {hi, lo} of the root
+
+template <int N, int ALT, typename T, bool PRE = true> struct Twiddles {
+  using PL = PlanOf<N, ALT>;
+  using Rec = typename RootRec<T>::type;
+  static constexpr int slots(int S) {
+    return S == 0 ? 0 : (PL::E / PL::P.r[S]) * (PL::P.r[S] > 4 ? 2 : 1);
+  }
+  static constexpr int offset(int S) {
+    int o = 0;
+    for (int s = 0; s < S; ++s) o += slots(s);
+    return o;
+  }
+  static constexpr int kTotal = offset(PL::NST);
+  static constexpr bool kPreload = PRE && kTotal > 0 && kTotal <= 6;
+  Rec r[kPreload ? kTotal : 1];
+
+  __device__ __forceinline__ void load(const T* roots, int t) {
+    if constexpr (kPreload) load_stage<1>(reinterpret_cast<const Rec*>(roots), t);
+  }
+  template <int S> __device__ __forceinline__ void load_stage(const Rec* tab, int t) {
+    if constexpr (S < PL::NST) {
+      constexpr int R = PL::template R<S>;
+      constexpr int NS = PL::template NS<S>();
+      constexpr int NB = PL::E / R;
+      constexpr int TWS = N / (NS * R);
+      constexpr int SL = R > 4 ? 2 : 1;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int k = (t + PL::T * b) % NS;
+        r[offset(S) + b * SL] = tab[k * TWS];
+        if constexpr (R > 4) r[offset(S) + b * SL + 1] = tab[4 * k * TWS];
+      }
+      load_stage<S + 1>(tab, t);
+    }
+  }
+};
+
 // ---------------------------------------------------------------- one Stockham transform
 // FP32 transforms exchange real and imaginary parts in turn (half the LDS per block); FP64
 // transforms exchange whole complex values.
@@ -230,16 +290,18 @@ template <typename T> constexpr int kExchangeBytes = kSplitExchange<T> ? (int)si
 // thread's transform through addr(p), in units of one exchanged value (kExchangeBytes).
 // roots[j] = exp(-2πi j/N) (N entries; in FP32 each entry is a pair {hi, lo}: roots[2j] the
 // rounded root, roots[2j + 1] its rounding remainder, read as one 16-byte record).
-template <int N, int ALT, int DIR, int S, typename T, typename Addr>
+template <int N, int ALT, int DIR, int S, typename T, typename Addr, bool PRE>
 __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
-                                          int tw_mode) {
+                                          const Twiddles<N, ALT, T, PRE>& tw) {
   using PL = PlanOf<N, ALT>;
+  using TW = Twiddles<N, ALT, T, PRE>;
   constexpr int E = PL::E, TT = PL::T;
   constexpr int R = PL::template R<S>;
   constexpr int NS = PL::template NS<S>();
   constexpr int NB = E / R;             // butterflies per thread
   constexpr bool LAST = (S == PL::NST - 1);
   constexpr int TWS = N / (NS * R);     // root-table stride of this stage's twiddles
+  constexpr int SL = R > 4 ? 2 : 1;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int j = t + TT * b;
@@ -248,87 +310,47 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
     if constexpr (NS > 1) {
-      if constexpr (sizeof(x[0].x) == 4) {
-        const float4* r4 = reinterpret_cast<const float4*>(roots);
-        if (tw_mode == 1) {
-          // FP32, two loads per butterfly: w^k and w^4k as {hi, lo} records taken to FP64,
-          // the other powers by FP64 products, each split back into hi + lo for the
-          // compensated product
-          auto root = [&](int idx) {
-            const float4 q = r4[idx];
-            double2 d{(double)q.x + (double)q.z, (double)q.y + (double)q.w};
-            if constexpr (DIR > 0) d.y = -d.y;
-            return d;
-          };
-          const double2 w1 = root(k * TWS);
-          const double2 w4 = R > 4 ? root(4 * k * TWS) : w1;
-          const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-#pragma unroll
-          for (int r = 1; r < R; ++r) {
-            double2 w;
-            if (r == 1) w = w1;
-            else if (r == 2) w = w2;
-            else if (r == 3) w = w3;
-            else if (r == 4) w = w4;
-            else if (r < 8) w = cmul(w4, r == 5 ? w1 : (r == 6 ? w2 : w3));
-            else {
-              const double2 w8 = cmul(w4, w4);
-              const int m = r & 3;
-              const double2 b = r < 12 ? w8 : cmul(w8, w4);
-              w = m == 0 ? b : cmul(b, m == 1 ? w1 : (m == 2 ? w2 : w3));
-            }
-            const float2 wh{(float)w.x, (float)w.y};
-            const float2 wl{(float)(w.x - (double)wh.x), (float)(w.y - (double)wh.y)};
-            x[r] = cmul_comp(x[r], wh, wl);
-          }
-        } else {
-          // FP32: every twiddle from the table as hi + lo, one 16-byte load per root
-#pragma unroll
-          for (int r = 1; r < R; ++r) {
-            const float4 w = r4[r * k * TWS];
-            T wh{w.x, w.y}, wl{w.z, w.w};
-            if constexpr (DIR > 0) {
-              wh = cconj(wh);
-              wl = cconj(wl);
-            }
-            x[r] = cmul_comp(x[r], wh, wl);
-          }
-        }
-      } else if (tw_mode == 0) {
-        // every twiddle from the FP64-built root table
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          T w = roots[r * k * TWS];
-          if constexpr (DIR > 0) w = cconj(w);
-          x[r] = cmul(x[r], w);
-        }
+      // w^k and w^4k (preloaded or read now) in FP64; the other powers by at most three FP64
+      // products. FP32 roots are {hi, lo} records taken to FP64, and every power is split back
+      // into hi + lo for the compensated product.
+      using Rec = typename TW::Rec;
+      const Rec* tab = reinterpret_cast<const Rec*>(roots);
+      auto as_root = [&](const Rec& q) {
+        double2 d;
+        if constexpr (sizeof(Rec) == 16 && sizeof(T) == 16) d = q;
+        else d = double2{(double)q.x + (double)q.z, (double)q.y + (double)q.w};
+        if constexpr (DIR > 0) d.y = -d.y;
+        return d;
+      };
+      double2 w1, w4;
+      if constexpr (TW::kPreload) {
+        w1 = as_root(tw.r[TW::offset(S) + b * SL]);
+        w4 = R > 4 ? as_root(tw.r[TW::offset(S) + b * SL + (SL - 1)]) : w1;
       } else {
-        // two table loads (w^k, w^4k); the other powers by at most three products
-        T w1 = roots[k * TWS];
-        if constexpr (DIR > 0) w1 = cconj(w1);
-        T w4 = w1;
-        if constexpr (R > 4) {
-          w4 = roots[4 * k * TWS];
-          if constexpr (DIR > 0) w4 = cconj(w4);
-        }
-        T w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-        T pw[4] = {T{}, w1, w2, w3};
+        w1 = as_root(tab[k * TWS]);
+        w4 = R > 4 ? as_root(tab[4 * k * TWS]) : w1;
+      }
+      const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          T w;
-          if (r < 4) w = pw[r];
-          else if (r == 4) w = w4;
-          else if (r < 8) w = cmul(w4, pw[r - 4]);
-          else {
-            T w8 = cmul(w4, w4);
-            if (r == 8) w = w8;
-            else if (r < 12) w = cmul(w8, pw[r - 8]);
-            else {
-              T w12 = cmul(w8, w4);
-              w = (r == 12) ? w12 : cmul(w12, pw[r - 12]);
-            }
-          }
+      for (int r = 1; r < R; ++r) {
+        double2 w;
+        if (r == 1) w = w1;
+        else if (r == 2) w = w2;
+        else if (r == 3) w = w3;
+        else if (r == 4) w = w4;
+        else if (r < 8) w = cmul(w4, r == 5 ? w1 : (r == 6 ? w2 : w3));
+        else {
+          const double2 w8 = cmul(w4, w4);
+          const int m = r & 3;
+          const double2 bb = r < 12 ? w8 : cmul(w8, w4);
+          w = m == 0 ? bb : cmul(bb, m == 1 ? w1 : (m == 2 ? w2 : w3));
+        }
+        if constexpr (sizeof(T) == 16) {
           x[r] = cmul(x[r], w);
+        } else {
+          const float2 wh{(float)w.x, (float)w.y};
+          const float2 wl{(float)(w.x - (double)wh.x), (float)(w.y - (double)wh.y)};
+          x[r] = cmul_comp(x[r], wh, wl);
         }
       }
     }
@@ -346,10 +368,10 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
       for (int r = 0; r < R; ++r) lds[addr(base + r * NS)] = v[b + NB * r];
     }
-    __syncthreads();
+    block_sync();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = lds[addr(t + TT * q)];
-    __syncthreads();
+    block_sync();
   }
   if constexpr (!LAST && kSplitExchange<T>) {
     // Exchange in two halves, real parts then imaginary parts: a block needs half the LDS, so
@@ -364,10 +386,10 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
       for (int r = 0; r < R; ++r) lr[addr(base + r * NS)] = v[b + NB * r].x;
     }
-    __syncthreads();
+    block_sync();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q].x = lr[addr(t + TT * q)];   // the outputs' .y stay put
-    __syncthreads();
+    block_sync();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int j = t + TT * b;
@@ -375,28 +397,28 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
       for (int r = 0; r < R; ++r) lr[addr(base + r * NS)] = v[b + NB * r].y;
     }
-    __syncthreads();
+    block_sync();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q].y = lr[addr(t + TT * q)];
-    __syncthreads();
+    block_sync();
   }
 }
 
-template <int N, int ALT, int DIR, int S, typename T, typename Addr>
+template <int N, int ALT, int DIR, int S, typename T, typename Addr, bool PRE>
 __device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
-                                           int tw_mode) {
+                                           const Twiddles<N, ALT, T, PRE>& tw) {
   if constexpr (S < PlanOf<N, ALT>::NST) {
-    fft_stage<N, ALT, DIR, S>(v, t, lds, addr, roots, tw_mode);
-    fft_stages<N, ALT, DIR, S + 1>(v, t, lds, addr, roots, tw_mode);
+    fft_stage<N, ALT, DIR, S>(v, t, lds, addr, roots, tw);
+    fft_stages<N, ALT, DIR, S + 1>(v, t, lds, addr, roots, tw);
   }
 }
 
-// tw_mode 0: all twiddles from the table; 1: two loads and products per butterfly (FP64 only;
-// FP32 always reads hi + lo twiddles from the table).
-template <int N, int ALT, int DIR, typename T, typename Addr>
+// `roots` is the root table of length N (FP64 roots, or FP32 {hi, lo} records); `tw` holds this
+// thread's twiddles when the plan preloads them (Twiddles::load at the start of the pass).
+template <int N, int ALT, int DIR, typename T, typename Addr, bool PRE>
 __device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
-                                    int tw_mode) {
-  fft_stages<N, ALT, DIR, 0>(v, t, lds, addr, roots, tw_mode);
+                                    const Twiddles<N, ALT, T, PRE>& tw) {
+  fft_stages<N, ALT, DIR, 0>(v, t, lds, addr, roots, tw);
 }
 
 }  // namespace qsf

@@ -75,7 +75,6 @@ struct PassParams {
   int64_t outer;         // prod_{d<axis} n_d
   const void* roots;     // N roots exp(-2πi j/N)
   uint32_t flags;
-  int32_t tw_mode;       // twiddles: 0 = table load each, 1 = two loads + products
   double scale;
   XFactor xa, xb;
   const void* pf[3];     // P factor tables (n_a entries; nullptr = 1)
@@ -95,7 +94,6 @@ struct TransferParams {
   void* dst;                 // receiver state (n0, n1)
   int64_t n0, n1;
   int32_t axis;              // freed axis f: 0 = column pass, 1 = row pass
-  int32_t tw_mode;
   const void* roots;         // roots of length n_f
   int64_t nCAP;
   const void* edge_in;       // source edge nodes along f

@@ -90,9 +90,6 @@ struct qsf_ctx {
   std::vector<hipEvent_t> spare;
   struct Entry { std::string key; int64_t n; double ms, bytes; };
   std::vector<Entry> entries;
-  // twiddles: two loads (w^k, w^4k) and FP64 products per butterfly, in both precisions
-  // (FP32 splits each power into hi + lo; measured faster than one {hi, lo} load per twiddle)
-  int tw_mode[2] = {1, 1};
   bool pair_onfly = false;  // EckhardSacha pair phase per element instead of the n*n table
   // kernel variant per (precision, row mode, axis length, grid size), chosen by the autotuner
   struct TuneKey {
@@ -513,7 +510,6 @@ struct Grid {
   // final sums and need no reduction launch)
   int run(PassParams& p, int force_variant = -1, double* single_out = nullptr) {
     build_tables();
-    p.tw_mode = ctx->tw_mode[prec];
     const int var = force_variant >= 0 ? force_variant : ctx->variant_of(prec, p.inner == 1, n[p.axis], m);
     if (p.flags & F_OBS) {
       int64_t tiles = axis_pass_tiles(prec, p, var);
@@ -669,7 +665,6 @@ struct Grid {
   double average_into(int R, int op, double* scr) {
     PassParams p = params(D - 1, psi.p, psi.p, false);
     p.flags = F_OBS | F_NOSTORE;
-    p.tw_mode = ctx->tw_mode[prec];
     obs_params(p.obs, op == 1);
     if (op == 2) {
       int64_t maxn = *std::max_element(n, n + 3);
@@ -1002,7 +997,6 @@ struct Set {
       tp.n0 = rg.n[0];
       tp.n1 = rg.n[1];
       tp.axis = r.axis;
-      tp.tw_mode = ctx->tw_mode[rg.prec];
       tp.roots = rg.root(rg.n[r.axis]);
       tp.nCAP = rg.d.nCAP;
       tp.edge_in = r.in->p;
@@ -1757,7 +1751,6 @@ struct Batch {
       rp.batch = G.entries.as<BatchEntry>();
       rp.step_ctr = ctr.as<int64_t>();
       rp.nbatch = (int32_t)G.members.size();
-      rp.tw_mode = ctx->tw_mode[prec];
       rp.flags = F_XB | F_FWD;
       rp.xb = G.xh;
       if (!first) {
@@ -1773,7 +1766,6 @@ struct Batch {
       cp.batch = G.entries.as<BatchEntry>();
       cp.step_ctr = ctr.as<int64_t>();
       cp.nbatch = (int32_t)G.members.size();
-      cp.tw_mode = ctx->tw_mode[prec];
       cp.flags = F_FWD | F_P | F_INV2;
       k = launch_axis_pass(prec, cp, st(), ctx->variant_of(prec, 0, L.n[0], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
@@ -1883,8 +1875,6 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
   if (!out) return QSF_ERR_ARG;
   auto* c = new qsf_ctx();
   c->device = device;
-  if (const char* e = std::getenv("QSF_TW_MODE")) c->tw_mode[0] = c->tw_mode[1] = std::atoi(e);
-  if (const char* e = std::getenv("QSF_TW_MODE_F32")) c->tw_mode[1] = std::atoi(e);
   if (const char* e = std::getenv("QSF_PAIR_ONFLY")) c->pair_onfly = std::atoi(e) != 0;
   int rc = guard(nullptr, [&] {
     int count = 0;

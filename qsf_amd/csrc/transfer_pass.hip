@@ -78,12 +78,14 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, 0>::THREADS))
   }
 
   const T* roots = reinterpret_cast<const T*>(p.roots);
+  Twiddles<N, 0, T> tw;
+  tw.load(roots, t);
   if constexpr (ROW) {
     RowAddr<T> a{w * row_stride<T, N>()};
-    fft<N, 0, -1>(v, t, lds, a, roots, p.tw_mode);
+    fft<N, 0, -1>(v, t, lds, a, roots, tw);
   } else {
     ColAddr<T, W> a{w};
-    fft<N, 0, -1>(v, t, lds, a, roots, p.tw_mode);
+    fft<N, 0, -1>(v, t, lds, a, roots, tw);
   }
 
   T* dst = reinterpret_cast<T*>(p.dst);
