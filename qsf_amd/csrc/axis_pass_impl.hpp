@@ -28,8 +28,11 @@ template <typename Real> struct Tr {
 // unpadded, 16 lanes of one write pass then hit two 4-byte-bank groups (FP64, R = 8: 128 B
 // apart) — an 8-way conflict. One pad element per 256 B of line (16 complex FP64 or 32 complex
 // FP32, one full sweep of the 64 banks) spreads those lanes over distinct banks.
-template <typename T> struct RowAddr {
+// WAVE: the block's transforms are each held by threads of one wave (rows of at most 64
+// threads), so their exchanges need no block barrier.
+template <typename T, bool WAVE = false> struct RowAddr {
   static constexpr int SH = sizeof(T) == 16 ? 4 : 5;
+  static constexpr bool kWave = WAVE;
   int base;
   __device__ __forceinline__ int operator()(int p) const { return base + p + (p >> SH); }
 };
@@ -46,10 +49,15 @@ constexpr int var_wshift(int v) { return v % 3 == 0 ? 0 : (v % 3 == 1 ? -1 : 1);
 // contains, so passes that only transform and multiply by the P factor (the column passes, the
 // middle passes, plain transforms) run a build without the X stage, the observation, the flux
 // gather, the region mask and the multigrid edge writes: far fewer registers, more waves.
-enum PassFeat : int { FEAT_FULL = 0, FEAT_LEAN = 1 };
+// Row passes that carry the X halves and an observation but no region mask, multigrid edges or
+// kinetic-energy copy (the 3D step's last-axis pass) run a middle tier without those paths.
+enum PassFeat : int { FEAT_FULL = 0, FEAT_LEAN = 1, FEAT_X = 2 };
 constexpr uint32_t kLeanFlags = F_INV1 | F_FWD | F_P | F_INV2 | F_SCALE | F_NOSTORE;
-inline int feat_of(const PassParams& p) {
-  return ((p.flags & ~kLeanFlags) == 0 && !p.er_out) ? FEAT_LEAN : FEAT_FULL;
+constexpr uint32_t kXFlags = kLeanFlags | F_XA | F_XB | F_OBS;
+inline int feat_of(const PassParams& p, bool row) {
+  if ((p.flags & ~kLeanFlags) == 0 && !p.er_out) return FEAT_LEAN;
+  if (row && (p.flags & ~kXFlags) == 0 && !p.er_out && !((p.flags & F_OBS) && p.obs.copy_out)) return FEAT_X;
+  return FEAT_FULL;
 }
 
 // Block geometry per (precision, N, mode, variant).
@@ -87,18 +95,36 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
   }
   static constexpr int W = pick();
   static constexpr int THREADS = W * T;
+  // Rows held one per wave (or several per wave): their exchanges need no block barrier.
+  static constexpr bool WAVE_LINES = ROW && T <= 64 && 64 % T == 0;
+  // Flux gather of a row pass: line w's complex values at byte gather_off, w * gather_stride
+  // elements in. FP64 lines lie inside their own exchange rows (so a line's LDS is only ever
+  // touched by its own threads); FP32 exchanges hold one real component, so the lines follow the
+  // exchange area.
+  static constexpr int kAlign(int b) { return (b + 15) & ~15; }
+  static constexpr bool kWholeExchange = kExchangeBytes<typename cvec<Real>::T> == CB;
+  static constexpr int gather_stride = kWholeExchange ? row_stride<typename cvec<Real>::T, N>() : N;
+  static constexpr int gather_off = kWholeExchange ? 0 : kAlign(lds_cap(W));
+  static constexpr int gather_end = gather_off + W * gather_stride * CB;
+  // per-wave observation sums (3 per wave) after everything else
+  static constexpr int NWAVES = (THREADS + 63) / 64;
+  static constexpr int red_off = kAlign(ROW && gather_end > lds_cap(W) ? gather_end : lds_cap(W));
+  static constexpr int red_bytes = 3 * NWAVES * (int)sizeof(double);
 };
 
 // Column-mode exchange layout: element p of column w at p*W + w, padded like the rows (with
 // narrow tiles, W = 2 for 2048-point FP64 columns, stage-0 writes of 16 lanes otherwise share
 // one 32-byte bank window).
 template <typename T, int W> struct ColAddr {
+  static constexpr bool kWave = false;
   int w;
   __device__ __forceinline__ int operator()(int p) const {
     const int a = p * W + w;
     return a + (a >> RowAddr<T>::SH);
   }
 };
+
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
 
 template <typename T>
 __device__ __forceinline__ T ld(const void* tab, int64_t i) {
@@ -107,21 +133,8 @@ __device__ __forceinline__ T ld(const void* tab, int64_t i) {
 
 // Per-thread evaluator of an X factor for the elements of one transform along `axis`
 // (scalar members only: runtime-indexed arrays would live in scratch memory).
-// Factors are complex FP64 in both precisions (tables, products, closed forms); the state
-// meets them once per element through capply (one rounding to the state type).
-// exp(-i v) (RE) or exp(-v) (IM) for an FP64 argument.
-__device__ __forceinline__ double2 expop(double v, int mode) {
-  if (mode == 1) return double2{exp(-v), 0.0};   // QSF_MODE_IM
-  double sn, cs;
-  sincos(-v, &sn, &cs);
-  return double2{cs, sn};
-}
-
-__device__ __forceinline__ double ee_closed(double x, double y, int model, double EE, double Esoft) {
-  if (model == 3) return EE / sqrt((x - y) * (x - y) + x * y + Esoft);  // EckhardSacha
-  return EE / sqrt((x - y) * (x - y) + Esoft);                          // Eberly
-}
-
+// Factors are complex FP64 in both precisions (tables and their products); the state meets them
+// once per element through capply (one rounding to the state type).
 template <int N> struct XEval {
   using T = double2;
   T cst;
@@ -129,10 +142,6 @@ template <int N> struct XEval {
   const T* pr0;
   const T* pr1;
   int o0, o1, npr, kind;
-  // closed-form pair parameters (copied: keeping a pointer into the kernel argument would
-  // force the whole argument block into scratch memory)
-  double delta, EE, Esoft, xmin, dx;
-  int mode, model;
   __device__ void init(const XFactor& f, int axis, int i0, int i1, int i2) {
     auto id = [&](int d) { return d == 0 ? i0 : (d == 1 ? i1 : i2); };
     cst = T{1.0, 0.0};
@@ -144,32 +153,7 @@ template <int N> struct XEval {
     o0 = o1 = 0;
     pr0 = pr1 = nullptr;
     kind = f.pair_kind;
-    delta = f.delta;
-    EE = f.EE;
-    Esoft = f.Esoft;
-    xmin = f.xmin;
-    dx = f.dx;
-    mode = f.mode;
-    model = f.model;
-    if (kind == PAIR_ONFLY) {
-      double sc = 0.0;   // pairs not involving the transform axis: one phase per thread
-      bool any = false;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (!(f.pair_mask & (1 << k))) continue;
-        const int p = k, q = (k + 1) % 3;
-        if (p != axis && q != axis) {
-          sc += ee_closed(xmin + id(p) * dx, xmin + id(q) * dx, model, EE, Esoft);
-          any = true;
-        } else {
-          const int o = (p == axis) ? id(q) : id(p);
-          if (npr == 0) o0 = o;
-          else o1 = o;
-          ++npr;
-        }
-      }
-      if (any) cst = cmul(cst, expop(delta * sc, mode));
-    } else if (kind != PAIR_NONE) {
+    if (kind != PAIR_NONE) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (!(f.pair_mask & (1 << k))) continue;
@@ -190,13 +174,6 @@ template <int N> struct XEval {
   // product of the pair terms that involve the transform axis (1 if none)
   __device__ __forceinline__ T pair_at(int n) const {
     T r{1.0, 0.0};
-    if (kind == PAIR_ONFLY) {
-      if (npr == 0) return r;
-      const double xn = xmin + n * dx;
-      double sv = ee_closed(xn, xmin + o0 * dx, model, EE, Esoft);
-      if (npr > 1) sv += ee_closed(xn, xmin + o1 * dx, model, EE, Esoft);
-      return expop(delta * sv, mode);
-    }
     if (npr > 0) r = kind == PAIR_DIFF ? pr0[abs(n - o0)] : pr0[n];
     if (npr > 1) r = cmul(r, kind == PAIR_DIFF ? pr1[abs(n - o1)] : pr1[n]);
     return r;
@@ -302,18 +279,20 @@ __device__ __forceinline__ int64_t tile_at(const PassParams& p, int64_t k, int64
 }
 
 // One tile of the pass on the state `v` already loaded: everything after the load, through the
-// store. Block sums of an observation are added to acc[] by thread 0.
+// store. An observation leaves each wave's sums in the reduction area of LDS.
 template <typename Real, int N, bool ROW, int V, int FEAT>
 __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, ROW, Geo<Real, N, ROW, V>::W>& g,
                                           typename Tr<Real>::T (&v)[PlanOf<N, var_alt(V)>::E], int tid, int t, int w,
                                           unsigned char* smem_raw, typename Tr<Real>::T* dst, const void* pf0,
-                                          const void* pf1, const void* pf2, void* gather_out, double* acc) {
+                                          const void* pf1, const void* pf2, void* gather_out) {
   using T = typename Tr<Real>::T;
+  using G = Geo<Real, N, ROW, V>;
   constexpr int ALT = var_alt(V);
   using PL = PlanOf<N, ALT>;
   constexpr int E = PL::E, TT = PL::T;
   constexpr int W = Geo<Real, N, ROW, V>::W;
   T* lds = reinterpret_cast<T*>(smem_raw);
+  T* glds = reinterpret_cast<T*>(smem_raw + G::gather_off);
   const T* roots = reinterpret_cast<const T*>(p.roots);
   const int64_t o = g.o, base = g.base, step = g.step;
   const bool valid = g.valid;
@@ -325,7 +304,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
   Twiddles<N, ALT, T, FEAT == FEAT_LEAN> tw;
   if (p.flags & (F_INV1 | F_FWD | F_INV2)) tw.load(roots, t);
   int gk0 = 0, gk1 = 0;
-  if (FEAT == FEAT_FULL && ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && valid) {
+  if (FEAT != FEAT_LEAN && ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && valid) {
     gk0 = p.obs.gather_rowptr[o];
     gk1 = p.obs.gather_rowptr[o + 1];
   }
@@ -340,7 +319,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
   auto run_fft = [&](auto dir_tag) {
     constexpr int DIR = decltype(dir_tag)::value;
     if constexpr (ROW) {
-      RowAddr<T> a{w * row_stride<T, N>()};
+      RowAddr<T, Geo<Real, N, ROW, V>::WAVE_LINES> a{w * row_stride<T, N>()};
       fft<N, ALT, DIR>(v, t, lds, a, roots, tw);
     } else {
       ColAddr<T, W> a{w};
@@ -354,7 +333,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
 
   // X stage: A factor -> observables / region mask -> B factor, one loop over the elements.
   // When both halves use the same pair-interaction table its lookup is shared.
-  const uint32_t XF = FEAT == FEAT_FULL ? p.flags & (F_XA | F_OBS | F_MASK | F_XB) : 0u;
+  const uint32_t XF = FEAT != FEAT_LEAN ? p.flags & (F_XA | F_OBS | F_MASK | F_XB) : 0u;
   if (XF) {
     const bool doA = p.flags & F_XA, doB = p.flags & F_XB, obs = p.flags & F_OBS, msk = p.flags & F_MASK;
     XEval<N> xa, xb;
@@ -394,25 +373,49 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
       na = p.axis == 0 ? p.n[0] : (p.axis == 1 ? p.n[1] : p.n[2]);
     }
     const bool gather = ROW && obs && ob.gather_rowptr;
-    if (!obs && !msk && !(ROW && p.er_out)) {
-      // common case (no observation this pass): compact loop, both halves per element
+    const bool gline = gather && gk1 > gk0;   // this line has border points to gather
+    // The state between the halves is formed explicitly only where something acts on it or keeps
+    // it (the region mask, the multigrid edges, the copy for the kinetic energy). Otherwise both
+    // halves meet in one product per element, the observables weight |psi|^2 by |A factor|^2,
+    // and the flux gather keeps the state before A and applies A at the points it reads.
+    const bool compact = FEAT == FEAT_X || (!msk && !(ROW && p.er_out) && !(obs && ob.copy_out));
+    if (compact) {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int ia = t + TT * q;
+        double2 zab{1.0, 0.0};
+        double wa = 1.0;   // |A factor|^2
         if (same) {
           const double2 z = cmul(xa.base_at(ia), xa.pair_at(ia));
-          v[q] = capply(v[q], cmul(z, z));
+          zab = cmul(z, z);
+          if (obs) wa = cabs2(z);
         } else if (shared) {
-          const double2 pp = xa.pair_at(ia);
-          v[q] = capply(v[q], cmul(cmul(xa.base_at(ia), xb.base_at(ia)), cmul(pp, pp)));
-        } else if (doA && doB) {
-          v[q] = capply(v[q], cmul(xa.at(ia), xb.at(ia)));
-        } else {
-          if (doA) v[q] = capply(v[q], xa.at(ia));
-          if (doB) v[q] = capply(v[q], xb.at(ia));
+          const double2 pp = xa.pair_at(ia), ba = xa.base_at(ia);
+          zab = cmul(cmul(ba, xb.base_at(ia)), cmul(pp, pp));
+          if (obs) wa = cabs2(ba) * cabs2(pp);
+        } else if (doA) {
+          const double2 za = xa.at(ia);
+          zab = doB ? cmul(za, xb.at(ia)) : za;
+          if (obs) wa = cabs2(za);
+        } else if (doB) {
+          zab = xb.at(ia);
         }
+        if (obs && valid) {
+          const double a2 = ((double)v[q].x * (double)v[q].x + (double)v[q].y * (double)v[q].y) * wa;
+          nrm += a2;
+          if (ob.want_epot) {
+            double co[3];
+            co[0] = ob.xmin[0] + (p.axis == 0 ? ia : i0) * ob.dx[0];
+            co[1] = ob.xmin[1] + (p.axis == 1 ? ia : i1) * ob.dx[1];
+            co[2] = ob.xmin[2] + (p.axis == 2 ? ia : i2) * ob.dx[2];
+            ep += a2 * pot_value(ob, co, D);
+          }
+          if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[ia] : 0.0));
+        }
+        if (gline) glds[w * G::gather_stride + ia] = v[q];
+        if (doA || doB) v[q] = capply(v[q], zab);
       }
-    } else
+    } else if constexpr (FEAT == FEAT_FULL) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int ia = t + TT * q;
@@ -435,7 +438,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
         if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[ia] : 0.0));
         if (ob.copy_out) reinterpret_cast<T*>(ob.copy_out)[base + (int64_t)ia * step] = v[q];
       }
-      if (gather) lds[w * N + ia] = v[q];
+      if (gline) glds[w * G::gather_stride + ia] = v[q];
       if (ROW && p.er_out && valid) {
         // multigrid source edges of the boundary state (2D main region, row i0 = o)
         const int64_t nc = p.nCAP, n0 = p.n[0], n1 = N;
@@ -462,31 +465,32 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
       }
       if (doB) v[q] = capply(v[q], same ? za : (shared ? cmul(xb.base_at(ia), pp) : xb.at(ia)));
     }
+    }
     if (obs) {
-      // block reduction (fixed order: lanes, then waves)
+      // wave sums of the requested observables (fixed order); the kernel adds the waves' sums
+      // once the tile is done
+      const bool want_ek = ob.kin_w[0] || ob.kin_w[1] || ob.kin_w[2];
       for (int off = 32; off > 0; off >>= 1) {
         nrm += __shfl_xor(nrm, off);
-        ep += __shfl_xor(ep, off);
-        ek += __shfl_xor(ek, off);
+        if (ob.want_epot) ep += __shfl_xor(ep, off);
+        if (want_ek) ek += __shfl_xor(ek, off);
       }
-      block_sync();
-      if (gather && valid) {
-        T* go = reinterpret_cast<T*>(gather_out);
-        for (int k = gk0 + t; k < gk1; k += TT) go[k] = lds[w * N + ob.gather_pos[k]];
+      if (gather) {
+        exchange_sync<G::WAVE_LINES>();
+        if (valid) {
+          T* go = reinterpret_cast<T*>(gather_out);
+          for (int k = gk0 + t; k < gk1; k += TT) {
+            const int pos = ob.gather_pos[k];
+            T val = glds[w * G::gather_stride + pos];
+            if (compact && doA) val = capply(val, xa.at(pos));   // the same product as a full pass
+            go[k] = val;
+          }
+        }
+        exchange_sync<G::WAVE_LINES>();   // the line's LDS is written again by the next exchange
       }
-      block_sync();
-      double* red = reinterpret_cast<double*>(smem_raw);
-      const int wave = tid >> 6, nwaves = (blockDim.x + 63) >> 6;
+      double* red = reinterpret_cast<double*>(smem_raw + G::red_off);
+      const int wave = tid >> 6;
       if ((tid & 63) == 0) { red[3 * wave] = nrm; red[3 * wave + 1] = ep; red[3 * wave + 2] = ek; }
-      block_sync();
-      if (tid == 0) {
-        double a = 0.0, e2 = 0.0, e3 = 0.0;
-        for (int k = 0; k < nwaves; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
-        acc[0] += a;
-        acc[1] += e2;
-        acc[2] += e3;
-      }
-      block_sync();
     }
   }
 
@@ -516,21 +520,23 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
 
 // Waves per SIMD the register allocation must leave room for. Left alone, the scheduler issues
 // every table load of the unrolled element loop at once and a full kernel lands at 169-171
-// VGPRs: two waves per SIMD, where 168 would give three (measured: the 512^3 row pass 2.4 ms at
-// three waves, 3.6 ms at two). Plans with 8 elements per thread fit three waves (full) or four
-// (lean, with its twiddles preloaded; the column tiles' LDS allows no more) without spilling,
-// 16-element plans two (full) or three (lean); the 24-element plans of the 3*2^k lengths need
-// their registers and are left uncapped.
-template <int E, int FEAT>
+// VGPRs: two waves per SIMD, where 168 would give three. Plans with 8 elements per thread fit
+// four waves in the lean tier (twiddles preloaded; the column tiles' LDS allows no more) and in
+// full row passes of 64k threads per line (their line geometry is wave-uniform, in scalar
+// registers; a few spilled
+// bytes cost far less than the fourth wave returns: the 512^3 row pass 2.6 ms at three waves,
+// 1.9 ms at four), three in full column passes; 16-element plans two (full) or three (lean);
+// the 24-element plans of the 3*2^k lengths need their registers and are left uncapped.
+template <int E, int FEAT, bool UNIFORM_LINE>
 constexpr int pass_waves() {
-  if (E <= 8) return FEAT == FEAT_LEAN ? 4 : 3;
-  if (E <= 16) return FEAT == FEAT_LEAN ? 3 : 2;
+  if (E <= 8) return (FEAT == FEAT_LEAN || UNIFORM_LINE) ? 4 : 3;
+  if (E <= 16) return FEAT == FEAT_LEAN ? 3 : 2;   // (the X tier counts as full)
   return 1;
 }
 
 template <typename Real, int N, bool ROW, int V, int FEAT>
 __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
-                                  (pass_waves<PlanOf<N, var_alt(V)>::E, FEAT>()))
+                                  (pass_waves<PlanOf<N, var_alt(V)>::E, FEAT, ROW && PlanOf<N, var_alt(V)>::T % 64 == 0>()))
     axis_pass_kernel(PassParams p) {
   using T = typename Tr<Real>::T;
   constexpr int ALT = var_alt(V);
@@ -541,7 +547,10 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
 
   const int tid = threadIdx.x;
   const int t = ROW ? (tid % TT) : (tid / W);
-  const int w = ROW ? (tid / TT) : (tid % W);
+  // rows of 64k threads: a wave holds part of one line, so the line index and everything
+  // derived from it (offsets, coordinates, table rows, constant factors) is wave-uniform and
+  // lives in scalar registers
+  const int w = ROW ? (TT % 64 == 0 ? __builtin_amdgcn_readfirstlane(tid / TT) : tid / TT) : (tid % W);
 
   const T* src = reinterpret_cast<const T*>(p.src);
   T* dst = reinterpret_cast<T*>(p.dst);
@@ -568,17 +577,24 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
   // (i % 8) * (nb / 8) + i / 8 and each XCD walks a contiguous run of tiles (its L2 sees the
   // neighbours' halves of shared lines)
   const int64_t k0 = (nb & 7) == 0 ? (int64_t)(blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3) : (int64_t)blockIdx.x;
-  double acc[3] = {0.0, 0.0, 0.0};
   TileGeo<N, ROW, W> g;
   g.at(p, tile_at<W, ROW>(p, k0, ntiles), w);
   T v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = g.valid ? src[g.base + (int64_t)(t + TT * q) * g.step] : T{Real(0), Real(0)};
-  pass_tile<Real, N, ROW, V, FEAT>(p, g, v, tid, t, w, smem_raw, dst, pf0, pf1, pf2, gather_out, acc);
-  if (FEAT == FEAT_FULL && (p.flags & F_OBS) && tid == 0 && partials) {
-    partials[3 * blockIdx.x] = acc[0];
-    partials[3 * blockIdx.x + 1] = acc[1];
-    partials[3 * blockIdx.x + 2] = acc[2];
+  pass_tile<Real, N, ROW, V, FEAT>(p, g, v, tid, t, w, smem_raw, dst, pf0, pf1, pf2, gather_out);
+  if (FEAT != FEAT_LEAN && (p.flags & F_OBS)) {
+    // the block's sums: its waves' sums in wave order
+    using G = Geo<Real, N, ROW, V>;
+    block_sync();
+    if (tid == 0 && partials) {
+      const double* red = reinterpret_cast<const double*>(smem_raw + G::red_off);
+      double a = 0.0, e2 = 0.0, e3 = 0.0;
+      for (int k = 0; k < G::NWAVES; ++k) { a += red[3 * k]; e2 += red[3 * k + 1]; e3 += red[3 * k + 2]; }
+      partials[3 * blockIdx.x] = a;
+      partials[3 * blockIdx.x + 1] = e2;
+      partials[3 * blockIdx.x + 2] = e3;
+    }
   }
 }
 
@@ -592,24 +608,20 @@ inline int64_t launch_blocks(const PassParams& p) {
 template <typename Real, int N, bool ROW, int V, int FEAT>
 inline int launch_feat(const PassParams& p, hipStream_t s) {
   using G = Geo<Real, N, ROW, V>;
-  // LDS per launch: the block reduction (3 sums per wave), the transform exchange, and for a
-  // flux gather the complex lines themselves; a pointwise pass (a region whose axes all stay in
-  // P) needs only the reduction, so more blocks fit on a CU
-  const size_t red_bytes = 16 * 3 * sizeof(double);
-  const size_t xchg_bytes = (size_t)G::lds_cap(G::W);
-  const size_t gather_bytes = FEAT == FEAT_FULL ? (size_t)G::W * N * G::CB : 0;
+  // LDS per launch: the transform exchange; for a flux gather the complex lines (inside the
+  // exchange rows in FP64, after them in FP32); for an observation the waves' sums after both.
+  // A pointwise pass (a region whose axes all stay in P) without an observation needs none.
+  constexpr size_t most = FEAT != FEAT_LEAN ? (size_t)(G::red_off + G::red_bytes) : (size_t)G::lds_cap(G::W);
+  static_assert(most <= 163840, "a block's LDS exceeds the 160 KiB of a CU");
   // once per kernel (a function-local static initialises once, also under concurrent callers)
-  static const bool attr_set = [&] {
-    size_t most = red_bytes > xchg_bytes ? red_bytes : xchg_bytes;
-    if (ROW && gather_bytes > most) most = gather_bytes;
+  static const bool attr_set = [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V, FEAT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
     return true;
   }();
   (void)attr_set;
-  size_t lds = red_bytes;
-  if ((p.flags & (F_INV1 | F_FWD | F_INV2)) && xchg_bytes > lds) lds = xchg_bytes;
-  if (ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && gather_bytes > lds) lds = gather_bytes;
+  size_t lds = (p.flags & (F_INV1 | F_FWD | F_INV2)) ? (size_t)G::lds_cap(G::W) : 0;
+  if (FEAT != FEAT_LEAN && (p.flags & F_OBS)) lds = most;
   const unsigned ny = p.batch ? (unsigned)p.nbatch : 1u;
   const int64_t blocks = launch_blocks<Real, N, ROW, V>(p);
   hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V, FEAT>), dim3((unsigned)blocks, ny), dim3(G::THREADS), lds, s,
@@ -619,8 +631,12 @@ inline int launch_feat(const PassParams& p, hipStream_t s) {
 
 template <typename Real, int N, bool ROW, int V>
 inline int launch_one(const PassParams& p, hipStream_t s) {
-  return feat_of(p) == FEAT_LEAN ? launch_feat<Real, N, ROW, V, FEAT_LEAN>(p, s)
-                                 : launch_feat<Real, N, ROW, V, FEAT_FULL>(p, s);
+  const int f = feat_of(p, ROW);
+  if (f == FEAT_LEAN) return launch_feat<Real, N, ROW, V, FEAT_LEAN>(p, s);
+  if constexpr (ROW) {
+    if (f == FEAT_X) return launch_feat<Real, N, ROW, V, FEAT_X>(p, s);
+  }
+  return launch_feat<Real, N, ROW, V, FEAT_FULL>(p, s);
 }
 
 // Blocks of a launch, which is also the number of per-block observation sums it writes.

@@ -26,6 +26,22 @@ __device__ __forceinline__ void block_sync() {
   asm volatile("" ::: "memory");
 }
 
+// Exchange barrier of data that stays within one wave (a transform whose threads are all in one
+// wave). A wave's LDS instructions execute in issue order, so no hardware barrier is needed: the
+// wavefront-scope fences and the asm clobbers only keep the compiler from moving LDS accesses
+// across this point.
+__device__ __forceinline__ void wave_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  asm volatile("" ::: "memory");
+}
+template <bool WAVE> __device__ __forceinline__ void exchange_sync() {
+  if constexpr (WAVE) wave_sync();
+  else block_sync();
+}
+
 template <typename R> struct cvec;
 template <> struct cvec<double> { using T = double2; };
 template <> struct cvec<float> { using T = float2; };
@@ -368,10 +384,10 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
       for (int r = 0; r < R; ++r) lds[addr(base + r * NS)] = v[b + NB * r];
     }
-    block_sync();
+    exchange_sync<Addr::kWave>();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = lds[addr(t + TT * q)];
-    block_sync();
+    exchange_sync<Addr::kWave>();
   }
   if constexpr (!LAST && kSplitExchange<T>) {
     // Exchange in two halves, real parts then imaginary parts: a block needs half the LDS, so
@@ -386,10 +402,10 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
       for (int r = 0; r < R; ++r) lr[addr(base + r * NS)] = v[b + NB * r].x;
     }
-    block_sync();
+    exchange_sync<Addr::kWave>();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q].x = lr[addr(t + TT * q)];   // the outputs' .y stay put
-    block_sync();
+    exchange_sync<Addr::kWave>();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int j = t + TT * b;
@@ -397,10 +413,10 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
       for (int r = 0; r < R; ++r) lr[addr(base + r * NS)] = v[b + NB * r].y;
     }
-    block_sync();
+    exchange_sync<Addr::kWave>();
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q].y = lr[addr(t + TT * q)];
-    block_sync();
+    exchange_sync<Addr::kWave>();
   }
 }
 

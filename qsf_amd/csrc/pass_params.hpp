@@ -20,9 +20,9 @@ enum PassFlags : uint32_t {
   F_NOSTORE = 1u << 9   // read-only pass (observables only)
 };
 
-// Pair interaction part of an X factor: none, a table indexed by |i-j| (Eberly), a full n*n
-// table, or evaluated per element from the closed form (argument in FP64, one sincos).
-enum PairKind : int32_t { PAIR_NONE = 0, PAIR_DIFF = 1, PAIR_FULL = 2, PAIR_ONFLY = 3 };
+// Pair interaction part of an X factor: none, a table indexed by |i-j| (Eberly), or a full n*n
+// table (EckhardSacha; measured faster than one FP64 sincos per element).
+enum PairKind : int32_t { PAIR_NONE = 0, PAIR_DIFF = 1, PAIR_FULL = 2 };
 
 // X factor = prod_a axis[a][i_a] * prod_{active pairs (p,q)} pair(i_p, i_q)
 struct XFactor {
@@ -31,7 +31,8 @@ struct XFactor {
   int32_t pair_kind;
   int32_t pair_mask;     // bit0: (0,1), bit1: (1,2), bit2: (2,0)
   int64_t pair_n;        // row length of a PAIR_FULL table
-  // PAIR_ONFLY: exp(-iδ·Σ ee) (RE) or exp(-δ·Σ ee) (IM) with ee of potential.h:130-136
+  // what the pair table holds: exp(-iδ·ee) (RE) or exp(-δ·ee) (IM), ee of potential.h:130-136
+  // (identifies the table when two factors are compared)
   double delta, EE, Esoft, xmin, dx;
   int32_t mode, model;   // QSF_MODE_*, QSF_POT_EBERLY | QSF_POT_ECKHARDT_SACHA
 };

@@ -90,7 +90,6 @@ struct qsf_ctx {
   std::vector<hipEvent_t> spare;
   struct Entry { std::string key; int64_t n; double ms, bytes; };
   std::vector<Entry> entries;
-  bool pair_onfly = false;  // EckhardSacha pair phase per element instead of the n*n table
   // kernel variant per (precision, row mode, axis length, grid size), chosen by the autotuner
   struct TuneKey {
     int prec, row;
@@ -434,10 +433,6 @@ struct Grid {
         // depends on |i-j| only: an n-entry table stays in L1/L2
         f.pair = pair_table(delta, mode);
         f.pair_kind = PAIR_DIFF;
-      } else if (ctx->pair_onfly) {
-        // EckhardSacha closed form per element (FP64 argument, one sincos)
-        f.pair = nullptr;
-        f.pair_kind = PAIR_ONFLY;
       } else {
         // EckhardSacha ee(x,y) is not a difference kernel: n*n phase table (measured faster
         // than one FP64 sincos per element on MI355X)
@@ -1875,7 +1870,6 @@ int qsf_ctx_create(int device, qsf_ctx** out) {
   if (!out) return QSF_ERR_ARG;
   auto* c = new qsf_ctx();
   c->device = device;
-  if (const char* e = std::getenv("QSF_PAIR_ONFLY")) c->pair_onfly = std::atoi(e) != 0;
   int rc = guard(nullptr, [&] {
     int count = 0;
     QSF_HIP(hipGetDeviceCount(&count));
