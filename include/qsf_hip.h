@@ -57,8 +57,10 @@ int qsf_prof_entry(qsf_ctx* ctx, int i, char* name, int name_len, int64_t* launc
 
 /* Autotuner: times every compiled variant (tile width x elements per thread) of the axis
  * passes this grid's step uses (reps launches each, HIP events) and keeps the fastest for
- * (precision, row/column mode, axis length, grid size) in the context. Entries can be read back
- * and set again (e.g. from a cache file kept by the caller). */
+ * (precision, pass class, axis length, grid size) in the context. The class (`row` below) is
+ * 1 for a row pass (the contiguous last axis), 0 for a column pass of an inner axis and 2 for a
+ * column pass of the outermost axis. Entries can be read back and set again (e.g. from a cache
+ * file kept by the caller). */
 int qsf_autotune(qsf_grid* g, int reps);
 int qsf_tune_count(qsf_ctx* ctx, int* count);
 int qsf_tune_entry(qsf_ctx* ctx, int i, int* precision, int* row, int64_t* n, int64_t* grid_size,

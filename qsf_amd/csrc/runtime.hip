@@ -90,7 +90,8 @@ struct qsf_ctx {
   std::vector<hipEvent_t> spare;
   struct Entry { std::string key; int64_t n; double ms, bytes; };
   std::vector<Entry> entries;
-  // kernel variant per (precision, row mode, axis length, grid size), chosen by the autotuner
+  // kernel variant per (precision, pass class, axis length, grid size), chosen by the autotuner;
+  // the class is tune_class() of the pass
   struct TuneKey {
     int prec, row;
     int64_t n, m;
@@ -110,6 +111,13 @@ struct qsf_ctx {
     return e;
   }
 };
+
+// Tuning class of an axis pass: 1 = row mode (the contiguous last axis), 0 = column mode of an
+// inner axis, 2 = column mode of the outermost axis. The outermost axis of a 3D grid strides
+// n1*n2 elements, every element of a column in another page: wide tiles (more bytes per page
+// visit) win there, narrow ones on the inner axes (measured at 512^3 FP64: 1.23 ms at 16
+// columns vs 1.34 ms at 8 on axis 0, 0.95 ms at 4 vs 1.02 ms at 8 on axis 1).
+int tune_class(const qsf::PassParams& p) { return p.inner == 1 ? 1 : (p.outer == 1 ? 2 : 0); }
 
 // ---- scratch of qsf_average_noalloc ----
 // One array per device, held until the process ends: [0, 8) the three sums of the result,
@@ -505,7 +513,7 @@ struct Grid {
   // final sums and need no reduction launch)
   int run(PassParams& p, int force_variant = -1, double* single_out = nullptr) {
     build_tables();
-    const int var = force_variant >= 0 ? force_variant : ctx->variant_of(prec, p.inner == 1, n[p.axis], m);
+    const int var = force_variant >= 0 ? force_variant : ctx->variant_of(prec, tune_class(p), n[p.axis], m);
     if (p.flags & F_OBS) {
       int64_t tiles = axis_pass_tiles(prec, p, var);
       partials.ensure(tiles * 3 * sizeof(double));
@@ -538,8 +546,8 @@ struct Grid {
     aux.ensure(m * cb);
     QSF_HIP(hipMemcpyAsync(aux.p, psi.p, m * cb, hipMemcpyDeviceToDevice, st()));
     const double C[3] = {0.01, 0.01, 0.01};
-    // the passes of this grid's fused step; passes that share a tuning key (same mode and
-    // length: the two column passes of a cubic 3D grid) are timed together per variant
+    // the passes of this grid's fused step; passes that share a tuning key (same class and
+    // length) are timed together per variant
     struct Cand { int axis; uint32_t flags; };
     std::vector<Cand> cands;
     if (D == 3 && fftAxes.size() == 3) {
@@ -555,7 +563,7 @@ struct Grid {
     QSF_HIP(hipEventCreate(&e0));
     QSF_HIP(hipEventCreate(&e1));
     std::map<std::pair<int, int64_t>, std::vector<Cand>> keyed;   // (row, n) -> passes
-    for (const Cand& c : cands) keyed[{c.axis == D - 1 ? 1 : 0, n[c.axis]}].push_back(c);
+    for (const Cand& c : cands) keyed[{tune_class(params(c.axis, aux.p, aux.p)), n[c.axis]}].push_back(c);
     for (const auto& kv : keyed) {
       const int row = kv.first.first;
       const int64_t len = kv.first.second;
@@ -665,7 +673,7 @@ struct Grid {
       int64_t maxn = *std::max_element(n, n + 3);
       for (int a = 0; a < D; ++a) p.obs.kin_w[a] = kinw.as<double>() + a * maxn;
     }
-    const int var = ctx->variant_of(prec, p.inner == 1, n[p.axis], m);
+    const int var = ctx->variant_of(prec, tune_class(p), n[p.axis], m);
     const int64_t tiles = axis_pass_tiles(prec, p, var);
     QSF_REQUIRE(tiles >= 1 && tiles <= kScratchMaxTiles, QSF_ERR_ARG,
                 "the observation pass of this grid has " + std::to_string(tiles) + " blocks; the "
@@ -1649,7 +1657,7 @@ struct Batch {
       L.build_tables();
       if (p.nborders > 0) L.build_gather(p.borders, p.nborders);
       PassParams rp = L.params(1, L.psi.p, L.psi.p);
-      G.tiles_row = axis_pass_tiles(prec, rp, ctx->variant_of(prec, 1, L.n[1], L.m));
+      G.tiles_row = axis_pass_tiles(prec, rp, ctx->variant_of(prec, tune_class(rp), L.n[1], L.m));
       std::vector<BatchEntry> be(G.members.size());
       std::vector<BatchObsArgs> oa(G.members.size());
       for (size_t k = 0; k < G.members.size(); ++k) {
@@ -1754,7 +1762,7 @@ struct Batch {
         L.obs_params(rp.obs, false);
         if (pd.nborders > 0) L.set_gather(rp.obs);
       }
-      int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, 1, L.n[1], L.m));
+      int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, tune_class(rp), L.n[1], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
       if (!first) launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), st());
       PassParams cp = L.params(0, L.psi.p, L.psi.p);
@@ -1762,7 +1770,7 @@ struct Batch {
       cp.step_ctr = ctr.as<int64_t>();
       cp.nbatch = (int32_t)G.members.size();
       cp.flags = F_FWD | F_P | F_INV2;
-      k = launch_axis_pass(prec, cp, st(), ctx->variant_of(prec, 0, L.n[0], L.m));
+      k = launch_axis_pass(prec, cp, st(), ctx->variant_of(prec, tune_class(cp), L.n[0], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
     }
     launch_step_increment(ctr.as<int64_t>(), st());
@@ -1945,7 +1953,7 @@ int qsf_tune_entry(qsf_ctx* c, int i, int* prec, int* row, int64_t* n, int64_t* 
 }
 
 int qsf_tune_set(qsf_ctx* c, int prec, int row, int64_t n, int64_t m, int variant) {
-  if (!c || variant < 0 || variant >= qsf::axis_variants(n)) return QSF_ERR_ARG;
+  if (!c || row < 0 || row > 2 || variant < 0 || variant >= qsf::axis_variants(n)) return QSF_ERR_ARG;
   c->tune[{prec, row, n, m}] = {variant, 0.0};
   return QSF_OK;
 }

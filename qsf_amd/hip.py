@@ -166,7 +166,9 @@ class Context:
         return out
 
     def tune_table(self):
-        """[(precision, row, n, grid_size, variant, best_ms)] chosen by the autotuner."""
+        """[(precision, row, n, grid_size, variant, best_ms)] chosen by the autotuner; `row` is the
+        pass class: 1 row pass, 0 column pass of an inner axis, 2 column pass of the outermost
+        axis."""
         n = C.c_int()
         self.check(load().qsf_tune_count(self.h, C.byref(n)), "tune_count")
         out = []
