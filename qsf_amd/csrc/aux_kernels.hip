@@ -94,6 +94,40 @@ __global__ void pair_table_kernel(PairTableArgs a) {
   reinterpret_cast<T*>(a.out)[k] = T{(Real)re, (Real)im};
 }
 
+template <typename Real>
+__global__ void radial_x_kernel(RadialArgs a) {
+  using T = typename cvec<Real>::T;
+  const int64_t m = a.n[0] * a.n[1] * a.n[2];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const int64_t id[3] = {e / (a.n[1] * a.n[2]), (e / a.n[2]) % a.n[1], e % a.n[2]};
+  // squares of the bounded coordinates folded from the right, as the parameter-pack fold of
+  // potential.h:43 does for the coordinates in order
+  double s = 0.0;
+  bool first = true;
+  for (int d = 2; d >= 0; --d) {
+    if (!(a.bounded_mask & (1 << d))) continue;
+    const double x = a.xmin[d] + id[d] * a.dx[d];
+    s = first ? x * x : (x * x) + s;
+    first = false;
+  }
+  const double v = a.delta * (a.NE / sqrt(s + a.Nsoft));
+  double re, im;
+  if (a.mode == QSF_MODE_IM) { re = exp(-v); im = 0.0; }
+  else { sincos(-v, &im, &re); }
+  T* p = reinterpret_cast<T*>(a.psi);
+  const T z = p[e];
+  const double zr = (double)z.x, zi = (double)z.y;
+  p[e] = T{(Real)(zr * re - zi * im), (Real)(zr * im + zi * re)};
+}
+
+void launch_radial_x(const RadialArgs& a, hipStream_t s) {
+  const int64_t m = a.n[0] * a.n[1] * a.n[2];
+  const dim3 grid((unsigned)((m + 255) / 256));
+  if (a.precision == QSF_FP64) hipLaunchKernelGGL(radial_x_kernel<double>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(radial_x_kernel<float>, grid, dim3(256), 0, s, a);
+}
+
 void launch_pair_table(int precision, const PairTableArgs& a, hipStream_t s) {
   const int64_t total = a.kind == 1 ? a.n : a.n * a.n;
   dim3 grid((unsigned)((total + 255) / 256));

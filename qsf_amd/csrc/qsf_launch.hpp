@@ -53,6 +53,19 @@ struct PairTableArgs {
 };
 void launch_pair_table(int precision, const PairTableArgs& a, hipStream_t s);
 
+// psi *= exp(-i delta V) (RE) or exp(-delta V) (IM) with V = NE / sqrt(sum x_d^2 + Nsoft) over the
+// bounded coordinates (CoulombInteraction, potential.h:40-44), in place on an X-rep state.
+struct RadialArgs {
+  void* psi;
+  int precision;
+  int64_t n[3];
+  double xmin[3], dx[3];
+  int bounded_mask;
+  double NE, Nsoft, delta;
+  int mode;
+};
+void launch_radial_x(const RadialArgs& a, hipStream_t s);
+
 // roots[j] = exp(-2πi j/n), computed in FP64 and rounded to the precision.
 void launch_roots(int precision, void* out, int64_t n, hipStream_t s);
 

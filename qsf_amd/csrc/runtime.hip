@@ -182,6 +182,10 @@ struct Grid {
   double xmin[3], dx[3], dp[3], dV, dVm, inv_m, eta = 0;
   int freeMask, group;
   bool bounded[3], isMain, many;
+  // CoulombInteraction over two or more bounded coordinates (potential.h:40-44) does not
+  // separate into per-axis factors: its X factor is applied by launch_radial_x after the
+  // per-axis tables (which then carry the coupling and absorber only)
+  bool radial = false;
   std::vector<int> fftAxes;
   int pperm[3];
   DBuf psi, aux, partials, red, redscratch, kinw, masktab, invtab, corrtab, tabs;
@@ -235,9 +239,7 @@ struct Grid {
     for (int i = 0; i < D; ++i)
       QSF_REQUIRE(axis_size_supported(n[i]), QSF_ERR_UNSUPPORTED,
                   "axis length " + std::to_string(n[i]) + " has no compiled FFT (2^k or 3*2^k, 8..4096)");
-    if (d.potential == QSF_POT_COULOMB)
-      QSF_REQUIRE(D == 1 || group == D - 1 || group == D, QSF_ERR_UNSUPPORTED,
-                  "CoulombInteraction is supported on the device for one bounded coordinate");
+    radial = d.potential == QSF_POT_COULOMB && D - group >= 2;
     if (d.potential == QSF_POT_EBERLY || d.potential == QSF_POT_ECKHARDT_SACHA)
       for (int a = 0; a < D; ++a)
         for (int b = a + 1; b < D; ++b)
@@ -412,7 +414,7 @@ struct Grid {
       t.xmin = xmin[a];
       t.dx = dx[a];
       t.use_ne = (d.potential != QSF_POT_NONE) && bounded[a] &&
-                 (reduced() || (d.potential == QSF_POT_COULOMB));
+                 (reduced() || (d.potential == QSF_POT_COULOMB && !radial));
       t.NE = d.Ncharge * d.Echarge;
       t.Nsoft = d.Nsoft;
       t.absorber = d.absorber;
@@ -638,6 +640,24 @@ struct Grid {
       p.flags = F_P;
     }
     run(p);
+    if (R == QSF_REP_X && radial) {
+      RadialArgs r{};
+      r.psi = psi.p;
+      r.precision = prec;
+      for (int a = 0; a < 3; ++a) {
+        r.n[a] = n[a];
+        r.xmin[a] = xmin[a];
+        r.dx[a] = dx[a];
+      }
+      r.bounded_mask = 0;
+      for (int a = 0; a < D; ++a)
+        if (bounded[a]) r.bounded_mask |= 1 << a;
+      r.NE = d.Ncharge * d.Echarge;
+      r.Nsoft = d.Nsoft;
+      r.delta = delta;
+      r.mode = mode;
+      launch_radial_x(r, st());
+    }
   }
 
   // Partial sums [norm, epot, ekin] of the current state, reduced on the device.
@@ -1139,8 +1159,9 @@ struct Prop {
     pend.resize(s->g.size());
     fluxint.ensure(16 * sizeof(double));
     QSF_HIP(hipMemsetAsync(fluxint.p, 0, 16 * sizeof(double), s->ctx->stream));
-    // observables that change the state or steer the loop every step run unfused
-    if (p.mode == QSF_MODE_IM || (p.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF))) exec = 1;
+    // observables that change the state or steer the loop every step run unfused, and so does
+    // a non-separable potential (its X factor is a pass of its own)
+    if (p.mode == QSF_MODE_IM || (p.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF)) || radial_set()) exec = 1;
     if (p.nborders > 0) main().build_gather(p.borders, p.nborders);
     s->props.push_back(this);
     s->active = this;
@@ -1148,6 +1169,11 @@ struct Prop {
 
   Grid& main() { return *set->g[0]; }
   hipStream_t st() { return ctx->stream; }
+  bool radial_set() const {
+    for (const auto& g : set->g)
+      if (g->radial) return true;
+    return false;
+  }
   int couples() const { return coup.rep(); }
 
   // ---- host side of makeStep: chain bookkeeping, precalc side effects, substep list ----
@@ -1622,6 +1648,8 @@ struct Batch {
                   "a batch holds single-region 2D grids of one precision");
       Solve& so = solves[s];
       so.g.reset(new Grid(c, d));
+      QSF_REQUIRE(!so.g->radial, QSF_ERR_UNSUPPORTED,
+                  "batched solves take separable potentials (CoulombInteraction in 2D runs as a single solve)");
       // host schedule: SplitPropagator RE loop of Strang steps (propagator.h:153, 311-346, 443-451)
       phys::Coupling coup(QSF_GAUGE_VELOCITY, &fields[s], 1);
       so.max_steps = (int64_t)(coup.max_pulse_duration() / p.dt + 1);
@@ -2394,6 +2422,8 @@ int qsf_prop_set_exec(qsf_prop* p, int mode) {
     require_set(p);
     QSF_REQUIRE(mode == 1 || !(p->p.pd.mode == QSF_MODE_IM || (p->p.pd.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF))),
                 QSF_ERR_UNSUPPORTED, "imaginary time, Normalize and ENERGY_DIFFERENCE run unfused");
+    QSF_REQUIRE(mode == 1 || !p->p.radial_set(), QSF_ERR_UNSUPPORTED,
+                "CoulombInteraction over two or more bounded coordinates runs unfused");
     p->p.flush();
     p->p.exec = mode;
   });

@@ -97,6 +97,11 @@ CASES = [
          Esoft=0.33, absorber=T.ABS_CAP, nCAP=8),
     dict(dim=3, n=32, dx=0.3, potential=T.POT_ECKHARDT_SACHA, Ncharge=3, Echarge=-1, Nsoft=1,
          Esoft=1, absorber=T.ABS_CAP, nCAP=6),
+    # CoulombInteraction over every coordinate (no per-axis separation: potential.h:40-44)
+    dict(dim=2, n=64, dx=0.2, potential=T.POT_COULOMB, Ncharge=2, Echarge=-1, Nsoft=0.5,
+         absorber=T.ABS_CAP, nCAP=8),
+    dict(dim=3, n=32, dx=0.3, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1,
+         absorber=T.ABS_COS18, nCAP=6, absorber_tau=0.05),
 ]
 
 
