@@ -68,7 +68,11 @@ int qsf_tune_entry(qsf_ctx* ctx, int i, int* precision, int* row, int64_t* n, in
 int qsf_tune_set(qsf_ctx* ctx, int precision, int row, int64_t n, int64_t grid_size, int variant);
 
 /* ---- grid sets: one grid, or the 2^D regions of MultiRegionsReduced (mpi_logic.h:246-263) ---- */
-/* Replaces LocalGrid(Section&) construction (grid.h:115-123, multigrid.h:45-46, 124-202). */
+/* Replaces LocalGrid(Section&) construction (grid.h:115-123, multigrid.h:45-46, 124-202).
+ * Multigrid sets: 2D (4 regions) or 3D (8 regions; a cubic grid whose side is a multiple of nCAP
+ * and at least 2*nCAP, as the reference's transfer boxes assume, and a reduced-dimension
+ * potential). CoulombInteraction over two or more bounded coordinates is applied as a pass of
+ * its own, so propagators over such sets, and over 3D multigrid sets, run unfused. */
 int qsf_set_create(qsf_ctx* ctx, const qsf_grid_desc* desc, qsf_set** out);
 /* Sets and the propagators created on them may be destroyed in either order; a propagator whose
  * set is gone answers its scalar queries and qsf_prop_destroy, and QSF_ERR_STATE otherwise. */

@@ -364,6 +364,54 @@ void launch_slice_2d(int precision, const SliceArgs& a, hipStream_t s) {
   else hipLaunchKernelGGL(slice_2d_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
 }
 
+// getBox + maskSlice (multigrid.h:454-569) of a 3D receiver, every box at once: the freed
+// axis' edge nodes of the source, times inv_mask of the freed-axis node and, for group-1
+// receivers, mask_correction (:519-530) with its folds in the reference's order.
+template <typename Real>
+__global__ void slice_3d_kernel(SliceArgs a) {
+  using T = typename cvec<Real>::T;
+  const int64_t m = a.n0 * a.n1 * a.n2;
+  const int64_t n[3] = {a.n0, a.n1, a.n2};
+  const T* src = reinterpret_cast<const T*>(a.src);
+  T* out = reinterpret_cast<T*>(a.slice);
+  const int f = a.dirFree;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t idx[3] = {k / (a.n1 * a.n2), (k / a.n2) % a.n1, k % a.n2};
+    const int64_t i_f = idx[f], n_f = n[f];
+    T v{Real(0), Real(0)};
+    if (i_f < a.nCAP || i_f >= n_f - a.nCAP) {
+      int64_t node[3];
+      for (int d = 0; d < 3; ++d) {
+        const int64_t nd = idx[d] < n[d] / 2 ? -idx[d] : idx[d] - n[d] + 1;   // neg_dist_from_edge
+        const int64_t c = a.nCAP + nd;
+        node[d] = c < 0 ? 0 : c;
+      }
+      double w = a.inv_tab[node[f]];
+      if (a.m1_tab) {
+        double sum = 0.0;   // t0 + (t1 + (t2 + 0))
+        for (int d = 2; d >= 0; --d) sum = (d == f ? 0.0 : a.m1_tab[node[d]]) + sum;
+        double prod = 1.0;  // u0 * (u1 * u2)
+        for (int d = 2; d >= 0; --d) {
+          const double u = d == f ? 1.0 : a.m1_tab[node[d]];
+          prod = d == 2 ? u : u * prod;
+        }
+        const double onesixth = 1.0 / 6.0, onehalf = 0.5;
+        w = w * (onehalf * sum + onesixth * (1 - prod));
+      }
+      const T x = src[k];
+      v = T{(Real)((double)x.x * w), (Real)((double)x.y * w)};
+    }
+    out[k] = v;
+  }
+}
+
+void launch_slice_3d(int precision, const SliceArgs& a, hipStream_t s) {
+  const int64_t m = a.n0 * a.n1 * a.n2;
+  const int blocks = (int)((m + 255) / 256 < 16384 ? (m + 255) / 256 : 16384);
+  if (precision == QSF_FP64) hipLaunchKernelGGL(slice_3d_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(slice_3d_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
+}
+
 template <typename T>
 __global__ void add_kernel(T* dst, const T* src, int64_t m) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (int64_t)gridDim.x * blockDim.x) {

@@ -104,13 +104,17 @@ namespace qsf {
 struct SliceArgs {
   const void* src;
   void* slice;
-  int64_t n0, n1;
-  int32_t dirFree;       // 0 or 1
+  int64_t n0, n1, n2;    // n2 = 1 for 2D
+  int32_t dirFree;       // 0, 1 (2D, 3D) or 2 (3D)
   int64_t nCAP;
   const double* inv_tab; // inv_mask(node) for node in [0, nCAP]
-  const double* corr_tab;// mask_correction by the other axis' node, or nullptr (not group 1)
+  const double* corr_tab;// 2D: mask_correction by the other axis' node, or nullptr (not group 1)
+  const double* m1_tab;  // 3D: mask(node) for node in [0, nCAP] when the receiver is group 1
 };
 void launch_slice_2d(int precision, const SliceArgs& a, hipStream_t s);
+// 3D slices of every box at once: the boxes of multigrid.h:454-569 tile the sliding axis, and
+// the transform along the freed axis never crosses them.
+void launch_slice_3d(int precision, const SliceArgs& a, hipStream_t s);
 void launch_add(int precision, void* dst, const void* src, int64_t m, hipStream_t s);
 }  // namespace qsf
 

@@ -188,7 +188,7 @@ struct Grid {
   bool radial = false;
   std::vector<int> fftAxes;
   int pperm[3];
-  DBuf psi, aux, partials, red, redscratch, kinw, masktab, invtab, corrtab, tabs;
+  DBuf psi, aux, partials, red, redscratch, kinw, masktab, invtab, corrtab, m1tab, tabs;
   std::map<int64_t, std::unique_ptr<DBuf>> roots;
   std::map<std::pair<int64_t, int>, std::unique_ptr<DBuf>> pairs;
   std::unique_ptr<Gather> gather;
@@ -318,16 +318,19 @@ struct Grid {
       masktab.ensure(mt.size() * sizeof(double));
       QSF_HIP(hipMemcpy(masktab.p, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
       // single-node inv_mask and the 2D group-1 slice correction (multigrid.h:519-548)
-      std::vector<double> inv(c), corr(c);
+      std::vector<double> inv(c), corr(c), m1(c);
       for (int64_t k = 0; k < c; ++k) {
         int64_t nd = k;
-        inv[k] = 1.0 - mask_nodes(1, &nd);
+        m1[k] = mask_nodes(1, &nd);
+        inv[k] = 1.0 - m1[k];
         corr[k] = 1 - 0.5 * (1.0 * inv[k]);
       }
       invtab.ensure(c * sizeof(double));
       corrtab.ensure(c * sizeof(double));
+      m1tab.ensure(c * sizeof(double));
       QSF_HIP(hipMemcpy(invtab.p, inv.data(), c * sizeof(double), hipMemcpyHostToDevice));
       QSF_HIP(hipMemcpy(corrtab.p, corr.data(), c * sizeof(double), hipMemcpyHostToDevice));
+      QSF_HIP(hipMemcpy(m1tab.p, m1.data(), c * sizeof(double), hipMemcpyHostToDevice));
     }
   }
 
@@ -935,7 +938,7 @@ struct Grid {
 };
 
 // ----------------------------------------------------------------------------------------
-// Grid sets and the 2D multigrid transfer (multigrid.h:617-657, mpi_logic.h:103-263)
+// Grid sets and the multigrid transfer (multigrid.h:617-657, mpi_logic.h:103-263)
 // ----------------------------------------------------------------------------------------
 struct Source { int region; int fc; };
 struct Prop;
@@ -962,20 +965,35 @@ struct Set {
       g.emplace_back(new Grid(c, d));
       return;
     }
-    QSF_REQUIRE(D == 2, QSF_ERR_UNSUPPORTED, "device multigrid is implemented for 2D grids");
-    const int free2[4] = {QSF_AXIS_NO, QSF_AXIS_X, QSF_AXIS_Y, QSF_AXIS_XY};  // mpi_logic.h:256-259
-    for (int fa : free2) {
+    QSF_REQUIRE(D == 2 || D == 3, QSF_ERR_UNSUPPORTED, "multigrid regions need 2 or 3 dimensions");
+    if (D == 3) {
+      // the transfer boxes (multigrid.h:188, 297-309) tile axes 1 and 2 in steps of nCAP and
+      // are counted on axis 0: a cube whose side nCAP divides
+      QSF_REQUIRE(d.n[0] == d.n[1] && d.n[1] == d.n[2] && d.nCAP >= 1 && d.n[0] % d.nCAP == 0 &&
+                      2 * d.nCAP <= d.n[0],
+                  QSF_ERR_UNSUPPORTED, "3D multigrid needs a cubic grid whose side is a multiple of 2*nCAP or more");
+      // partial() of the YZ region keeps z (potential.h:72-77), a free coordinate there
+      QSF_REQUIRE(d.potential != QSF_POT_COULOMB, QSF_ERR_UNSUPPORTED,
+                  "3D multigrid takes the reduced-dimension interactions");
+    }
+    // MultiRegionsReduced region order (mpi_logic.h:249-263)
+    static const int free2[4] = {QSF_AXIS_NO, QSF_AXIS_X, QSF_AXIS_Y, QSF_AXIS_XY};
+    static const int free3[8] = {QSF_AXIS_NO, QSF_AXIS_X,  QSF_AXIS_Y,  QSF_AXIS_Z,
+                                 QSF_AXIS_XY, QSF_AXIS_XZ, QSF_AXIS_YZ, QSF_AXIS_XYZ};
+    const int* fr = D == 2 ? free2 : free3;
+    const int R = D == 2 ? 4 : 8;
+    for (int r = 0; r < R; ++r) {
       qsf_grid_desc dd = d;
-      dd.region_free = fa;
+      dd.region_free = fr[r];
       g.emplace_back(new Grid(c, dd));
     }
-    groupCount = 3;
-    sources.resize(4);
-    for (int r = 0; r < 4; ++r)
-      for (int i = 0; i < 4; ++i)
-        if (__builtin_popcount(free2[r]) - __builtin_popcount(free2[i]) == 1) {
-          const int x = free2[r] ^ free2[i];
-          if (x == QSF_AXIS_X || x == QSF_AXIS_Y) sources[r].push_back({i, x});
+    groupCount = D + 1;
+    sources.resize(R);
+    for (int r = 0; r < R; ++r)
+      for (int i = 0; i < R; ++i)
+        if (__builtin_popcount(fr[r]) - __builtin_popcount(fr[i]) == 1) {
+          const int x = fr[r] ^ fr[i];
+          if (x == QSF_AXIS_X || x == QSF_AXIS_Y || x == QSF_AXIS_Z) sources[r].push_back({i, x});
         }
   }
 
@@ -1061,11 +1079,17 @@ struct Set {
           a.slice = slice.p;
           a.n0 = rg.n[0];
           a.n1 = rg.n[1];
-          a.dirFree = s.fc == QSF_AXIS_X ? 0 : 1;
+          a.n2 = rg.n[2];
+          a.dirFree = s.fc == QSF_AXIS_X ? 0 : (s.fc == QSF_AXIS_Y ? 1 : 2);
           a.nCAP = rg.d.nCAP;
           a.inv_tab = rg.invtab.as<double>();
-          a.corr_tab = rg.group == 1 ? rg.corrtab.as<double>() : nullptr;
-          launch_slice_2d(rg.prec, a, ctx->stream);
+          if (D == 2) {
+            a.corr_tab = rg.group == 1 ? rg.corrtab.as<double>() : nullptr;
+            launch_slice_2d(rg.prec, a, ctx->stream);
+          } else {
+            a.m1_tab = rg.group == 1 ? rg.m1tab.as<double>() : nullptr;
+            launch_slice_3d(rg.prec, a, ctx->stream);
+          }
           PassParams p = rg.params(a.dirFree, slice.p, slice.p);
           p.flags = F_FWD;  // transf_slice: FFTW_FORWARD, unnormalised
           rg.run(p);
@@ -1161,7 +1185,9 @@ struct Prop {
     QSF_HIP(hipMemsetAsync(fluxint.p, 0, 16 * sizeof(double), s->ctx->stream));
     // observables that change the state or steer the loop every step run unfused, and so does
     // a non-separable potential (its X factor is a pass of its own)
-    if (p.mode == QSF_MODE_IM || (p.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF)) || radial_set()) exec = 1;
+    if (p.mode == QSF_MODE_IM || (p.outputs & (QSF_OUT_NORMALIZE | QSF_OUT_EDIFF)) || radial_set() ||
+        (s->multi && s->D == 3))
+      exec = 1;
     if (p.nborders > 0) main().build_gather(p.borders, p.nborders);
     s->props.push_back(this);
     s->active = this;
@@ -2424,6 +2450,8 @@ int qsf_prop_set_exec(qsf_prop* p, int mode) {
                 QSF_ERR_UNSUPPORTED, "imaginary time, Normalize and ENERGY_DIFFERENCE run unfused");
     QSF_REQUIRE(mode == 1 || !p->p.radial_set(), QSF_ERR_UNSUPPORTED,
                 "CoulombInteraction over two or more bounded coordinates runs unfused");
+    QSF_REQUIRE(mode == 1 || !(p->p.set->multi && p->p.set->D == 3), QSF_ERR_UNSUPPORTED,
+                "3D multigrid sets run unfused");
     p->p.flush();
     p->p.exec = mode;
   });
