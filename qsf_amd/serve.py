@@ -143,14 +143,19 @@ def bound_state(desc):
 
 def poisson_stream(count, rate=4.0, seed=5, mix=(0.60, 0.35, 0.05), verify_every=10, **shape_kw):
     """Seeded Poisson arrivals of mixed requests; every `verify_every`-th FP32-mixed request gets
-    an FP64 twin (same arrival) whose result checks it."""
+    an FP64 twin (same arrival) whose result checks it. The kinds hold the mix exactly (rounded
+    counts in a seeded random order), so a short stream still carries its share of 3D solves."""
     rng = np.random.default_rng(seed)
+    counts = [int(round(count * m)) for m in mix]
+    counts[int(np.argmax(counts))] += count - sum(counts)
+    kinds = np.repeat(np.array(["1d", "2d", "3d"]), counts)
+    rng.shuffle(kinds)
     t = 0.0
     out = []
     cache = {}
     for i in range(count):
         t += float(rng.exponential(1.0 / rate))
-        kind = rng.choice(["1d", "2d", "3d"], p=list(mix))
+        kind = kinds[i]
         d, p, f = request_shapes(kind, rng, **shape_kw)
         key = (d.dim, tuple(d.n), d.dx[0])
         if key not in cache:
