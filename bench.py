@@ -43,6 +43,70 @@ def c2_prop():
 C2_FIELDS = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
 
 
+def c1_desc(precision=T.FP64, n=4096):
+    return T.grid_desc(1, n, 0.1, precision=precision, potential=T.POT_COULOMB, Ncharge=1, Echarge=-1,
+                       Nsoft=1.0, absorber=T.ABS_COS18, nCAP=int(round(0.1 * n)), absorber_tau=0.05)
+
+
+C1_WORKLOAD = ("1D single electron N=4096 dx=0.1, soft-core Coulomb, cos^(1/8) on 10% of each edge: "
+               "imaginary-time ground state (2000 steps, dt=0.05, Normalize + energies every step), "
+               "then the real-time Strang run over the 4-cycle 800 nm sin^2 pulse (E0=0.0534, length "
+               "gauge, dt=0.05) with norm, energies and flux through |x|=150 a.u. every step")
+
+
+def run_c1(oracle_too=True):
+    """C1: the ground state by imaginary time, then the pulse in real time, each timed whole."""
+    from qsf_amd import hip
+    x = -0.5 * 0.1 * 4095 + 0.1 * np.arange(4096)
+    guess = (np.exp(-np.abs(x)) + 0j) / np.sqrt(np.sum(np.exp(-2 * np.abs(x))) * 0.1)
+    im = T.prop_desc(T.MODE_IM, 1, 0.05, max_steps=2000,
+                     outputs=T.OUT_NORMALIZE | T.OUT_EPOT | T.OUT_EKIN | T.OUT_ETOT | T.OUT_EDIFF)
+    re = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_LENGTH,
+                     outputs=T.OUT_NORM | T.OUT_EPOT | T.OUT_EKIN | T.OUT_ETOT | T.OUT_FLUX,
+                     borders=[(T.BORDER_BOX, 1500)])
+    fields = [T.field_desc(T.AXIS_X, 0.0534, 0.057, 4)]
+    ctx = hip.Context(0)
+    gs = hip.GridSet(ctx, c1_desc())
+    gs[0].set_psi(guess)
+    t0 = time.perf_counter()
+    p = hip.Propagator(gs, im, [T.field_desc(T.AXIS_X, 0.0, 0.057, 4)])
+    rim = p.run()
+    ctx.synchronize()
+    t_im = time.perf_counter() - t0
+    p.close()
+    t0 = time.perf_counter()
+    p = hip.Propagator(gs, re, fields)
+    rre = p.run()
+    ctx.synchronize()
+    t_re = time.perf_counter() - t0
+    flux = p.flux_integral()
+    p.close()
+    gs.close()
+    ctx.close()
+    out = dict(im_steps=int(rim.shape[0]), im_s=t_im, e0=float(rim[-1, T.REC_ETOT]),
+               re_steps=int(rre.shape[0]), re_s=t_re, yield_norm=float(1 - rre[-1, T.REC_NORM]),
+               flux_integral=float(flux[0]))
+    if oracle_too:
+        from oracle import oracle as O
+        progress("c1: the same two runs in the oracle (one host thread)")
+        O.set_threads(1)
+        g = O.GridSet(c1_desc())
+        g[0].set_psi(guess)
+        t0 = time.perf_counter()
+        oi = O.Propagator(g, im, [T.field_desc(T.AXIS_X, 0.0, 0.057, 4)]).run()
+        t_oim = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        po = O.Propagator(g, re, fields)
+        orr = po.run()
+        t_ore = time.perf_counter() - t0
+        out.update(oracle_im_s=t_oim, oracle_re_s=t_ore, oracle_e0=float(oi[-1, T.REC_ETOT]),
+                   e0_abs_diff=abs(float(oi[-1, T.REC_ETOT]) - out["e0"]),
+                   yield_rel_diff=abs(float(1 - orr[-1, T.REC_NORM]) - out["yield_norm"]) /
+                   max(abs(float(1 - orr[-1, T.REC_NORM])), 1e-300),
+                   oracle_flux_integral=float(po.flux_integral()[0]) if hasattr(po, "flux_integral") else None)
+    return out
+
+
 def c3_desc(precision, n=512):
     return T.grid_desc(3, n, 0.3, precision=precision, potential=T.POT_ECKHARDT_SACHA, Ncharge=3,
                        Echarge=-1, Nsoft=1.0, Esoft=1.0, absorber=T.ABS_CAP, nCAP=32)
@@ -379,7 +443,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5", "copy"], default="c3")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5", "copy"], default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="c3: leave out the batched 2D comparison")
     ap.add_argument("--requests", type=int, default=40, help="c5: number of requests in the stream")
@@ -392,6 +456,17 @@ def main():
 
     if args.workload == "copy":
         print(json.dumps(copy_calibration(args.steps)))
+        return
+    if args.workload == "c1":
+        r = run_c1(oracle_too=not args.no_cpu)
+        steps = r["im_steps"] + r["re_steps"]
+        wall = r["im_s"] + r["re_s"]
+        out = {"metric": "grid-point-steps/s", "value": round(4096 * steps / wall, 1),
+               "unit": "grid-point-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+               "ms_per_step": round(wall / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "c128 (f64)", "data": "synthetic",
+               "config": {"workload": C1_WORKLOAD}, "c1": r}
+        print(json.dumps(out))
         return
     if args.workload == "c5":
         r = run_c5(args.requests)
