@@ -33,6 +33,7 @@ template <typename Real> struct Tr {
 template <typename T, bool WAVE = false> struct RowAddr {
   static constexpr int SH = sizeof(T) == 16 ? 4 : 5;
   static constexpr bool kWave = WAVE;
+  static constexpr bool kSplitOk = false;
   int base;
   __device__ __forceinline__ int operator()(int p) const { return base + p + (p >> SH); }
 };
@@ -69,7 +70,7 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
   // at a time)
   static constexpr int lds_cap(int w) {
     using T = typename cvec<Real>::T;
-    return (ROW ? w * row_stride<T, N>() : w * N + ((w * N) >> RowAddr<T>::SH)) * kExchangeBytes<T>;
+    return ROW ? w * row_stride<T, N>() * (int)sizeof(T) : (w * N + ((w * N) >> RowAddr<T>::SH)) * kExchangeBytes<T>;
   }
   // the data the tile holds, which the occupancy budgets below are stated for
   static constexpr int lds_data(int w) { return w * N * CB; }
@@ -97,15 +98,12 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
   static constexpr int THREADS = W * T;
   // Rows held one per wave (or several per wave): their exchanges need no block barrier.
   static constexpr bool WAVE_LINES = ROW && T <= 64 && 64 % T == 0;
-  // Flux gather of a row pass: line w's complex values at byte gather_off, w * gather_stride
-  // elements in. FP64 lines lie inside their own exchange rows (so a line's LDS is only ever
-  // touched by its own threads); FP32 exchanges hold one real component, so the lines follow the
-  // exchange area.
+  // Flux gather of a row pass: line w's complex values in its own exchange row (rows exchange
+  // whole complex values), so a line's LDS is only ever touched by its own threads.
   static constexpr int kAlign(int b) { return (b + 15) & ~15; }
-  static constexpr bool kWholeExchange = kExchangeBytes<typename cvec<Real>::T> == CB;
-  static constexpr int gather_stride = kWholeExchange ? row_stride<typename cvec<Real>::T, N>() : N;
-  static constexpr int gather_off = kWholeExchange ? 0 : kAlign(lds_cap(W));
-  static constexpr int gather_end = gather_off + W * gather_stride * CB;
+  static constexpr int gather_stride = row_stride<typename cvec<Real>::T, N>();
+  static constexpr int gather_off = 0;
+  static constexpr int gather_end = W * gather_stride * CB;
   // per-wave observation sums (3 per wave) after everything else
   static constexpr int NWAVES = (THREADS + 63) / 64;
   static constexpr int red_off = kAlign(ROW && gather_end > lds_cap(W) ? gather_end : lds_cap(W));
@@ -117,6 +115,7 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
 // one 32-byte bank window).
 template <typename T, int W> struct ColAddr {
   static constexpr bool kWave = false;
+  static constexpr bool kSplitOk = true;
   int w;
   __device__ __forceinline__ int operator()(int p) const {
     const int a = p * W + w;

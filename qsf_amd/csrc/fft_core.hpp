@@ -301,6 +301,9 @@ template <int N, int ALT, typename T, bool PRE = true> struct Twiddles {
 // transforms exchange whole complex values.
 template <typename T> constexpr bool kSplitExchange = sizeof(T) == 8;
 template <typename T> constexpr int kExchangeBytes = kSplitExchange<T> ? (int)sizeof(T) / 2 : (int)sizeof(T);
+// Row layouts exchange whole complex values in both precisions (a row tile's LDS is small, and
+// each split exchange costs two more barriers); column tiles split FP32 exchanges.
+template <typename T, typename Addr> constexpr bool kSplitFor = kSplitExchange<T> && Addr::kSplitOk;
 
 // v[q] holds x[t + T*q] on entry and X[t + T*q] on exit. `lds` addresses element p of this
 // thread's transform through addr(p), in units of one exchanged value (kExchangeBytes).
@@ -374,7 +377,7 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
 #pragma unroll
     for (int r = 0; r < R; ++r) v[b + NB * r] = x[r];   // slot b + NB*r holds output r
   }
-  if constexpr (!LAST && !kSplitExchange<T>) {
+  if constexpr (!LAST && !kSplitFor<T, Addr>) {
     // complex exchange: output r of butterfly b goes to position (j/NS)*NS*R + j%NS + r*NS,
     // j = t + T*b; element t + T*q comes back
 #pragma unroll
@@ -389,7 +392,7 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
     for (int q = 0; q < E; ++q) v[q] = lds[addr(t + TT * q)];
     exchange_sync<Addr::kWave>();
   }
-  if constexpr (!LAST && kSplitExchange<T>) {
+  if constexpr (!LAST && kSplitFor<T, Addr>) {
     // Exchange in two halves, real parts then imaginary parts: a block needs half the LDS, so
     // twice as many blocks stay resident (measured: a gain for FP32 passes, a loss for FP64,
     // whose extra barriers cost more than the occupancy returns).
