@@ -525,7 +525,8 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
 // registers; a few spilled
 // bytes cost far less than the fourth wave returns: the 512^3 row pass 2.6 ms at three waves,
 // 1.9 ms at four), three in full column passes; 16-element plans two (full) or three (lean);
-// the 24-element plans of the 3*2^k lengths need their registers and are left uncapped.
+// the 20- and 24-element plans of the 5*2^k and 3*2^k lengths need their registers and are left
+// uncapped.
 template <int E, int FEAT, bool UNIFORM_LINE>
 constexpr int pass_waves() {
   if (E <= 8) return (FEAT == FEAT_LEAN || UNIFORM_LINE) ? 4 : 3;
@@ -684,6 +685,7 @@ int64_t tiles_size(int v, const PassParams& p) {
 }
 
 #define QSF_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) \
-  X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072)
+  X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072)                         \
+  X(10) X(20) X(40) X(80) X(160) X(320) X(640) X(1280) X(2560)
 
 }  // namespace qsf

@@ -1,6 +1,6 @@
 // axis_pass_inst.hip — explicit instantiations of the fused axis pass for one precision, one
 // mode and one group of axis lengths. The Makefile compiles this file once per
-// (QSF_PREC: 0 = double, 1 = float; QSF_ROWMODE: 1 = row, 0 = column; QSF_GROUP: 0..9), so the
+// (QSF_PREC: 0 = double, 1 = float; QSF_ROWMODE: 1 = row, 0 = column; QSF_GROUP: 0..11), so the
 // kernels of the long transforms build in parallel translation units.
 #include "axis_pass_impl.hpp"
 
@@ -34,6 +34,10 @@ using QsfReal = float;
 #define QSF_GROUP_SIZES(X) X(2048)
 #elif QSF_GROUP == 9
 #define QSF_GROUP_SIZES(X) X(4096)
+#elif QSF_GROUP == 10
+#define QSF_GROUP_SIZES(X) X(10) X(20) X(40) X(80) X(160) X(320)
+#elif QSF_GROUP == 11
+#define QSF_GROUP_SIZES(X) X(640) X(1280) X(2560)
 #else
 #error "QSF_GROUP out of range"
 #endif

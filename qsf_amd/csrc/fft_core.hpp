@@ -119,6 +119,25 @@ template <int DIR, typename T> __device__ __forceinline__ void dft3(T& a, T& b, 
   c = csub(m, r);
 }
 
+// DFT of length 5 with w = exp(DIR 2πi/5): conjugate pairs w^k, w^(5-k) share the real parts
+// cos(2π/5), cos(4π/5) on the sums b+e, c+d and the imaginary parts on the differences
+template <int DIR, typename T> __device__ __forceinline__ void dft5(T& a, T& b, T& c, T& d, T& e) {
+  constexpr double C1 = 0.30901699437494742410229341718282;    // cos(2π/5)
+  constexpr double C2 = -0.80901699437494742410229341718282;   // cos(4π/5)
+  constexpr double S1 = 0.95105651629515357211643933337938;    // sin(2π/5)
+  constexpr double S2 = 0.58778525229247312916870595463907;    // sin(4π/5)
+  const T t1 = cadd(b, e), t2 = cadd(c, d), t3 = csub(b, e), t4 = csub(c, d);
+  const T m1 = cadd(cadd(a, cscale_k(t1, C1)), cscale_k(t2, C2));
+  const T m2 = cadd(cadd(a, cscale_k(t1, C2)), cscale_k(t2, C1));
+  const T n1 = mul_di<DIR>(cadd(cscale_k(t3, S1), cscale_k(t4, S2)));
+  const T n2 = mul_di<DIR>(csub(cscale_k(t3, S2), cscale_k(t4, S1)));
+  a = cadd(cadd(a, t1), t2);
+  b = cadd(m1, n1);
+  e = csub(m1, n1);
+  c = cadd(m2, n2);
+  d = csub(m2, n2);
+}
+
 template <int DIR, typename T> __device__ __forceinline__ void dft4(T& a, T& b, T& c, T& d) {
   T s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = mul_di<DIR>(csub(b, d));
   a = cadd(s0, s2);
@@ -195,6 +214,7 @@ template <int R, int DIR, typename T> __device__ __forceinline__ void dft(T* v) 
   if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
   else if constexpr (R == 3) dft3<DIR>(v[0], v[1], v[2]);
   else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 5) dft5<DIR>(v[0], v[1], v[2], v[3], v[4]);
   else if constexpr (R == 8) dft8<DIR>(v);
   else dft16<DIR>(v);
 }
@@ -208,6 +228,7 @@ struct Plan {
 // Elements per thread. `alt` selects the other register/LDS balance for the same length
 // (fewer elements per thread: more threads, more LDS exchanges, fewer VGPRs; or the reverse).
 __host__ __device__ constexpr int plan_e(int n, int alt = 0) {
+  if (n % 5 == 0) return n <= 20 ? n : 20;   // 5·2^k: one radix-5 stage, then radix 4 / 2
   if (n % 3 == 0) {
     const int e = n <= 12 ? n : (n <= 24 ? n : 24);
     return (alt && e == 24 && n >= 48) ? 12 : e;
@@ -225,6 +246,7 @@ __host__ __device__ constexpr Plan make_plan(int n, int alt = 0) {
   p.t = n / p.e;
   int rem = n;
   if (rem % 3 == 0) { p.r[p.nst++] = 3; rem /= 3; }
+  if (rem % 5 == 0) { p.r[p.nst++] = 5; rem /= 5; }
   const int cand[4] = {16, 8, 4, 2};
   while (rem > 1) {
     for (int i = 0; i < 4; ++i) {

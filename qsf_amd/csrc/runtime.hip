@@ -238,7 +238,7 @@ struct Grid {
     if (isMain && D > 1) { pperm[0] = 1; pperm[1] = 0; }  // FFTW_MPI_TRANSPOSED_OUT P layout
     for (int i = 0; i < D; ++i)
       QSF_REQUIRE(axis_size_supported(n[i]), QSF_ERR_UNSUPPORTED,
-                  "axis length " + std::to_string(n[i]) + " has no compiled FFT (2^k or 3*2^k, 8..4096)");
+                  "axis length " + std::to_string(n[i]) + " has no compiled FFT (2^k 8..4096, 3*2^k 12..3072, 5*2^k 10..2560)");
     radial = d.potential == QSF_POT_COULOMB && D - group >= 2;
     if (d.potential == QSF_POT_EBERLY || d.potential == QSF_POT_ECKHARDT_SACHA)
       for (int a = 0; a < D; ++a)

@@ -41,7 +41,8 @@ def pair(ctx, desc):
 
 
 # ------------------------------------------------------------------------------------ FFT
-SIZES_1D = [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 12, 24, 48, 96, 192, 384, 768, 1536, 3072]
+SIZES_1D = [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 12, 24, 48, 96, 192, 384, 768, 1536, 3072,
+            10, 20, 40, 80, 160, 320, 640, 1280, 2560]
 
 
 @pytest.mark.parametrize("n", SIZES_1D)
@@ -60,7 +61,7 @@ def test_fft_1d(ctx, n, prec):
     assert rel(dev[0].psi(), x) < tol
 
 
-@pytest.mark.parametrize("shape", [(64, 48), (256, 256), (2048, 16), (12, 1024)])
+@pytest.mark.parametrize("shape", [(64, 48), (256, 256), (2048, 16), (12, 1024), (80, 320), (640, 40)])
 @pytest.mark.parametrize("prec", [T.FP64, T.FP32_MIXED])
 def test_fft_2d_transposed_layout(ctx, shape, prec):
     d = T.grid_desc(2, list(shape), [0.2, 0.3], precision=prec)
@@ -73,7 +74,7 @@ def test_fft_2d_transposed_layout(ctx, shape, prec):
     assert rel(dev[0].psi(), ref[0].psi()) < tol   # both in FFTW's (n1, n0) P layout
 
 
-@pytest.mark.parametrize("shape", [(16, 32, 8), (32, 32, 32), (64, 16, 128)])
+@pytest.mark.parametrize("shape", [(16, 32, 8), (32, 32, 32), (64, 16, 128), (40, 20, 80)])
 def test_fft_3d(ctx, shape):
     d = T.grid_desc(3, list(shape), [0.3, 0.3, 0.3])
     dev, ref = pair(ctx, d)
@@ -448,3 +449,36 @@ def test_cos18_edge_node_stays_finite(ctx):
     m = gs[0].psi().real.reshape(n, n)
     assert np.all(m[0, :] == 0.0) and np.all(m[:, -1] == 0.0) and np.isfinite(m).all()
     gs.close()
+
+
+# ------------------------------------------------------------------------------------ 5*2^k lengths
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32_MIXED])
+@pytest.mark.parametrize("multigrid", [0, 1])
+def test_propagation_radix5_lengths(ctx, prec, multigrid):
+    # 160 = 5 * 32: a radix-5 first stage, then radix 4 and 2 (the fused passes, and for the
+    # multigrid set the fused region transfers, at that length)
+    n, dx = 160, 0.25
+    d = T.grid_desc(2, n, dx, precision=prec, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5,
+                    Esoft=0.33, absorber=T.ABS_CAP, nCAP=16, multigrid=multigrid)
+    d64 = T.grid_desc(2, n, dx, potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                      absorber=T.ABS_CAP, nCAP=16, multigrid=multigrid)
+    psi0 = crand(n * n) * 0.01
+    if prec == T.FP32_MIXED:
+        psi0 = psi0.astype(np.complex64).astype(np.complex128)
+    f = [T.field_desc(T.AXIS_XY, 0.12, 0.057, 3)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.02, gauge=T.GAUGE_VELOCITY,
+                        outputs=T.OUT_NORM | T.OUT_FLUX, borders=[(b, 0) for b in T.ZOA_FLUX_2D])
+    from qsf_amd import hip
+    print(f"[case] radix-5 n={n} prec={prec} multigrid={multigrid}", flush=True)
+    dev = hip.GridSet(ctx, d)
+    ref = O.GridSet(d64)
+    for r in range(len(ref)):
+        z = psi0 if r == 0 else np.zeros_like(psi0)
+        dev[r].set_psi(z)
+        ref[r].set_psi(z)
+    rd = hip.Propagator(dev, pdesc, f).run(8)
+    ro = O.Propagator(ref, pdesc, f).run(8)
+    tol = FP64_TOL if prec == T.FP64 else 1e-4
+    for r in range(len(ref)):
+        assert rel(dev[r].psi(), ref[r].psi()) < tol, f"region {r}"
+    assert np.allclose(rd[:, 2], ro[:, 2], rtol=1e-11 if prec == T.FP64 else 1e-5)
