@@ -88,10 +88,14 @@ template <typename Real, int N, bool ROW, int V = 0> struct Geo {
     while (w > 1 && (lds_data(w) > 65536 || w * T > 1024)) w /= 2;
     return w;
   }
+  // Blocks of 1024 threads need four waves per SIMD, i.e. at most 128 VGPRs: plans with 16 or
+  // more elements per thread do not fit that without spilling, so their wide variant stops at
+  // 512 threads.
   static constexpr int pick() {
     int w = base();
+    const int most = PL::E >= 16 ? 512 : 1024;
     if (var_wshift(V) < 0 && w > 1) w /= 2;
-    if (var_wshift(V) > 0 && 2 * w * T <= 1024 && lds_cap(2 * w) <= 163840) w *= 2;
+    if (var_wshift(V) > 0 && 2 * w * T <= most && lds_cap(2 * w) <= 163840) w *= 2;
     return w;
   }
   static constexpr int W = pick();
