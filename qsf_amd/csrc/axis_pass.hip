@@ -67,4 +67,35 @@ int64_t axis_pass_tiles(int precision, const PassParams& p, int v) {
   return row ? tiles<float, true>(n, v, p) : tiles<float, false>(n, v, p);
 }
 
+template <typename Real>
+__global__ void gather_x_kernel(GatherXArgs a) {
+  using T = typename cvec<Real>::T;
+  const T* psi = reinterpret_cast<const T*>(a.psi);
+  T* out = reinterpret_cast<T*>(a.out);
+  const int axis = a.dim - 1;
+  const int64_t len = a.n[axis];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.nvals;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = a.row[k];
+    const int p = a.pos[k];
+    T v = psi[r * len + p];
+    if (a.has_xa) {
+      // the line's coordinates (axis `axis` varies along the row)
+      const int i0 = a.dim == 3 ? (int)(r / a.n[1]) : (a.dim == 2 ? (int)r : 0);
+      const int i1 = a.dim == 3 ? (int)(r % a.n[1]) : 0;
+      XEval<1> xa;
+      xa.init(a.xa, axis, i0, i1, 0);
+      v = capply(v, xa.at(p));
+    }
+    out[k] = v;
+  }
+}
+
+void launch_gather_x(int precision, const GatherXArgs& a, hipStream_t s) {
+  if (a.nvals <= 0) return;
+  const int blocks = (int)((a.nvals + 255) / 256 < 65536 ? (a.nvals + 255) / 256 : 65536);
+  if (precision == 0) hipLaunchKernelGGL(gather_x_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(gather_x_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
+}
+
 }  // namespace qsf

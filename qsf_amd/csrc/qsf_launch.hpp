@@ -11,6 +11,22 @@ bool axis_size_supported(int64_t n);
 // Launches one fused axis pass with kernel variant v (0 = default; see axis_variants);
 // returns the number of blocks (or -1 if n is unsupported).
 int launch_axis_pass(int precision, const PassParams& p, hipStream_t s, int v = 0);
+
+// The flux stencil values of an X-representation state ahead of the pass that would gather
+// them: out[k] = psi[row[k], pos[k]] times the pending X factor xa at that point (when has_xa),
+// the same product the pass forms. Rows run along the last axis.
+struct GatherXArgs {
+  const void* psi;
+  void* out;
+  const int32_t* row;
+  const int32_t* pos;
+  int64_t nvals;
+  int64_t n[3];
+  int32_t dim;
+  int32_t has_xa;
+  XFactor xa;
+};
+void launch_gather_x(int precision, const GatherXArgs& a, hipStream_t s);
 int64_t axis_pass_tiles(int precision, const PassParams& p, int v = 0);
 // Number of compiled variants for an axis length (tile width x elements per thread).
 int axis_variants(int64_t n);

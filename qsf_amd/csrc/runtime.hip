@@ -169,7 +169,7 @@ inline void drain_prof(qsf_ctx* c) {
 // ----------------------------------------------------------------------------------------
 struct Gather {        // sparse flux points of one border list (see build_gather)
   std::vector<qsf_border_desc> borders;
-  DBuf rowptr, pos, pts, bvec, values, partials, out;
+  DBuf rowptr, pos, rowidx, pts, bvec, values, partials, out;
   int64_t npts = 0, nvals = 0;
 };
 
@@ -813,10 +813,11 @@ struct Grid {
     need.erase(std::unique(need.begin(), need.end()), need.end());
     auto slot_of = [&](int64_t c) { return (int32_t)(std::lower_bound(need.begin(), need.end(), c) - need.begin()); };
     const int64_t rowlen = n[D - 1], rows = m / rowlen;
-    std::vector<int32_t> rowptr(rows + 1, 0), pos(need.size());
+    std::vector<int32_t> rowptr(rows + 1, 0), pos(need.size()), rowidx(need.size());
     for (size_t k = 0; k < need.size(); ++k) {
       rowptr[need[k] / rowlen + 1]++;
       pos[k] = (int32_t)(need[k] % rowlen);
+      rowidx[k] = (int32_t)(need[k] / rowlen);
     }
     for (int64_t r = 0; r < rows; ++r) rowptr[r + 1] += rowptr[r];
     std::vector<int32_t> pt(pts.size() * (1 + 2 * D));
@@ -839,12 +840,30 @@ struct Grid {
     };
     up(g->rowptr, rowptr.data(), rowptr.size() * 4);
     up(g->pos, pos.data(), pos.size() * 4);
+    up(g->rowidx, rowidx.data(), rowidx.size() * 4);
     up(g->pts, pt.data(), pt.size() * 4);
     up(g->bvec, bv.data(), bv.size() * 8);
     g->values.ensure(std::max<int64_t>(g->nvals, 1) * cb);
     g->partials.ensure(1024 * 16 * sizeof(double));
     g->out.ensure(16 * sizeof(double));
     gather = std::move(g);
+  }
+
+  // The stencil values of the X state ahead of the pass that observes it, with the pending X
+  // factor (or none) applied at each point: what that pass would gather, read from HBM instead
+  // of from the pass's registers (for passes whose input is the X state).
+  void gather_ahead(const XFactor* xa) {
+    GatherXArgs a{};
+    a.psi = psi.p;
+    a.out = gather->values.p;
+    a.row = gather->rowidx.as<int32_t>();
+    a.pos = gather->pos.as<int32_t>();
+    a.nvals = gather->nvals;
+    for (int k = 0; k < 3; ++k) a.n[k] = n[k];
+    a.dim = D;
+    a.has_xa = xa ? 1 : 0;
+    if (xa) a.xa = *xa;
+    launch_gather_x(prec, a, st());
   }
 
   void set_gather(ObsParams& o) {
@@ -1404,7 +1423,17 @@ struct Prop {
           p.flags |= F_XB;
           p.xb = fb;
           if (pend[r].on) { p.flags |= F_XA; p.xa = pend[r].f; }
-          if (observe) obs_into(g, p);
+          if (observe) {
+            obs_into(g, p);
+            if (p.obs.gather_rowptr) {
+              // the state is in X before this pass: its flux stencil is read ahead, so the pass
+              // does no gather (a row pass's gather costs it more than this read)
+              g.gather_ahead(pend[r].on ? &pend[r].f : nullptr);
+              p.obs.gather_rowptr = nullptr;
+              p.obs.gather_pos = nullptr;
+              p.obs.gather_out = nullptr;
+            }
+          }
           int tiles = g.run(p, -1, observe ? obs_row() : nullptr);
           if (observe) after_obs(g, tiles);
         } else {
