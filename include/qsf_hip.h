@@ -154,8 +154,9 @@ int qsf_prop_flux_integral(qsf_prop* p, double* out);
 /* 0 = fused axis passes (default), 1 = one standalone call per reference member. */
 int qsf_prop_set_exec(qsf_prop* p, int exec_mode);
 
-/* ---- batches: many single-region 2D solves (real-time Strang, velocity gauge, one field each)
- *      advanced together. One step = one batched launch per (size group, pass) driven by a
+/* ---- batches: many single-region 1D and 2D solves, mixed in one batch (real-time Strang,
+ *      velocity gauge, one field each, one border list for all) advanced together. One step =
+ *      one batched launch per (size group, pass: one pass for 1D, two for 2D) driven by a
  *      device step counter; with use_graph it is captured once and replayed as a hipGraph.
  *      Each solve follows its own SplitPropagator loop (max_steps+2 steps, propagator.h:153,
  *      443-451); finished solves drop out of the launches. ---- */

@@ -1699,8 +1699,8 @@ struct Batch {
     solves.resize(ns);
     for (int s = 0; s < ns; ++s) {
       const qsf_grid_desc& d = descs[s];
-      QSF_REQUIRE(d.dim == 2 && !d.multigrid && d.precision == prec, QSF_ERR_UNSUPPORTED,
-                  "a batch holds single-region 2D grids of one precision");
+      QSF_REQUIRE((d.dim == 1 || d.dim == 2) && !d.multigrid && d.precision == prec, QSF_ERR_UNSUPPORTED,
+                  "a batch holds single-region 1D and 2D grids of one precision");
       Solve& so = solves[s];
       so.g.reset(new Grid(c, d));
       QSF_REQUIRE(!so.g->radial, QSF_ERR_UNSUPPORTED,
@@ -1739,8 +1739,10 @@ struct Batch {
       G.xh = L.x_factor(QSF_MODE_RE, 0.5 * p.dt, zero, false);   // built once, never recycled
       L.build_tables();
       if (p.nborders > 0) L.build_gather(p.borders, p.nborders);
-      PassParams rp = L.params(1, L.psi.p, L.psi.p);
-      G.tiles_row = axis_pass_tiles(prec, rp, ctx->variant_of(prec, tune_class(rp), L.n[1], L.m));
+      // the pass that observes: the row pass of a 2D solve, the one pass of a 1D solve
+      const int ra = L.D - 1;
+      PassParams rp = L.params(ra, L.psi.p, L.psi.p);
+      G.tiles_row = axis_pass_tiles(prec, rp, ctx->variant_of(prec, tune_class(rp), L.n[ra], L.m));
       std::vector<BatchEntry> be(G.members.size());
       std::vector<BatchObsArgs> oa(G.members.size());
       for (size_t k = 0; k < G.members.size(); ++k) {
@@ -1754,11 +1756,11 @@ struct Batch {
         if (p.nborders > 0) so.gathered.ensure(std::max<int64_t>(L.gather->nvals, 1) * g.cb);
         char* pt = static_cast<char*>(so.ptab.p);
         const int64_t half = std::max(g.n[0], g.n[1]) * Grid::kTabBytes;
-        be[k] = BatchEntry{g.psi.p, {pt, pt + half, nullptr}, so.partials.as<double>(),
+        be[k] = BatchEntry{g.psi.p, {pt, g.D > 1 ? pt + half : nullptr, nullptr}, so.partials.as<double>(),
                            p.nborders > 0 ? so.gathered.p : nullptr, so.total};
         BatchTableArgs& t = ta[G.members[k]];
-        t = BatchTableArgs{};
-        for (int a = 0; a < 2; ++a) {
+        t = BatchTableArgs{};   // n[a] = 0 for the axes a 1D solve lacks: no table there
+        for (int a = 0; a < g.D; ++a) {
           const int sa = g.pperm[a];
           t.out[a] = a == 0 ? (void*)pt : (void*)(pt + half);
           t.n[a] = g.n[a];
@@ -1778,12 +1780,12 @@ struct Batch {
       G.oargs.ensure(oa.size() * sizeof(BatchObsArgs));
       QSF_HIP(hipMemcpy(G.oargs.p, oa.data(), oa.size() * sizeof(BatchObsArgs), hipMemcpyHostToDevice));
       G.geo = BatchFluxGeo{};
-      G.geo.D = 2;
+      G.geo.D = L.D;
       G.geo.precision = prec;
       G.geo.dt = p.dt;
       G.geo.nb = p.nborders;
       G.geo.scale = L.dV / L.dx[0];
-      for (int a = 0; a < 3; ++a) G.geo.inv_2dx[a] = a < 2 ? (1.0 / L.dx[a]) / 2.0 : 0.0;
+      for (int a = 0; a < 3; ++a) G.geo.inv_2dx[a] = a < L.D ? (1.0 / L.dx[a]) / 2.0 : 0.0;
       if (p.nborders > 0) {
         G.geo.pts = L.gather->pts.as<int32_t>();
         G.geo.bvec = L.gather->bvec.as<double>();
@@ -1833,21 +1835,25 @@ struct Batch {
     launch_batch_ptables(prec, targs.as<BatchTableArgs>(), (int)solves.size(), nmax(), ctr.as<int64_t>(), st());
     for (auto& G : groups) {
       Grid& L = *G.lead;
-      PassParams rp = L.params(1, L.psi.p, L.psi.p);
+      // 1D: one pass [X_a, OBS, X_b, FWD, P, INV] (the state stays in X between steps);
+      // 2D: row pass [INV, X_a, OBS, X_b, FWD] and column pass [FWD, P, INV]
+      const int ra = L.D - 1;
+      PassParams rp = L.params(ra, L.psi.p, L.psi.p);
       rp.batch = G.entries.as<BatchEntry>();
       rp.step_ctr = ctr.as<int64_t>();
       rp.nbatch = (int32_t)G.members.size();
-      rp.flags = F_XB | F_FWD;
+      rp.flags = F_XB | F_FWD | (L.D == 1 ? F_P | F_INV2 : 0u);
       rp.xb = G.xh;
       if (!first) {
-        rp.flags |= F_INV1 | F_XA | F_OBS;
+        rp.flags |= F_XA | F_OBS | (L.D == 1 ? 0u : F_INV1);
         rp.xa = G.xh;
         L.obs_params(rp.obs, false);
         if (pd.nborders > 0) L.set_gather(rp.obs);
       }
-      int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, tune_class(rp), L.n[1], L.m));
+      int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, tune_class(rp), L.n[ra], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
       if (!first) launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), st());
+      if (L.D == 1) continue;
       PassParams cp = L.params(0, L.psi.p, L.psi.p);
       cp.batch = G.entries.as<BatchEntry>();
       cp.step_ctr = ctr.as<int64_t>();
@@ -1886,8 +1892,8 @@ struct Batch {
   void finish() {
     for (auto& so : solves) {
       Grid& g = *so.g;
-      PassParams p = g.params(1, g.psi.p, g.psi.p);
-      p.flags = F_INV1 | F_XA;
+      PassParams p = g.params(g.D - 1, g.psi.p, g.psi.p);
+      p.flags = F_XA | (g.D == 1 ? 0u : F_INV1);
       p.xa = groups[so.group].xh;
       g.run(p);
       so.end_row = observe_now(so, so.total);

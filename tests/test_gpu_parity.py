@@ -425,6 +425,54 @@ def test_batch_matches_single_solves(ctx, use_graph):
     b.close()
 
 
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32_MIXED])
+def test_batch_mixed_dimensions(ctx, prec):
+    # 1D and 2D solves in one batch (one pass per step for the 1D groups, two for the 2D groups),
+    # uneven sizes including 3*2^k and 5*2^k, replayed as a hipGraph; each solve against its own
+    # FP64 oracle propagation
+    from qsf_amd import hip
+    specs = [(1, 256, dict(potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1.0,
+                           absorber=T.ABS_COS18, nCAP=26, absorber_tau=0.05), 0.1),
+             (1, 384, dict(potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1.0,
+                           absorber=T.ABS_CAP, nCAP=38), 0.1),
+             (2, 64, dict(potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                          absorber=T.ABS_CAP, nCAP=8), 0.3),
+             (1, 320, dict(potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1.0,
+                           absorber=T.ABS_CAP, nCAP=32), 0.1),
+             (2, 96, dict(potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+                          absorber=T.ABS_COS18, nCAP=10, absorber_tau=0.02), 0.3),
+             (1, 256, dict(potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1.0,
+                           absorber=T.ABS_COS18, nCAP=26, absorber_tau=0.05), 0.1)]
+    descs = [T.grid_desc(dim, n, dx, precision=prec, **kw) for dim, n, kw, dx in specs]
+    descs64 = [T.grid_desc(dim, n, dx, **kw) for dim, n, kw, dx in specs]
+    fields = [T.field_desc(T.AXIS_X if dim == 1 else T.AXIS_XY, 0.05 + 0.02 * i, 2.0, 1 + (i % 2),
+                           phase_in_pi=0.3 * i) for i, (dim, n, kw, dx) in enumerate(specs)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_VELOCITY, outputs=T.OUT_NORM | T.OUT_FLUX,
+                        borders=[(T.BORDER_BOX, 12)])
+    psis = []
+    for dim, n, kw, dx in specs:
+        x = crand(n ** dim) * 0.01
+        psis.append(x.astype(np.complex64).astype(np.complex128) if prec == T.FP32_MIXED else x)
+    b = hip.Batch(ctx, descs, fields, pdesc)
+    for s, p in enumerate(psis):
+        b.set_psi(s, p)
+    b.run_all(use_graph=True)
+    tol = FP64_TOL if prec == T.FP64 else 1e-4
+    for s, (dim, n, kw, dx) in enumerate(specs):
+        print(f"[case] mixed batch solve {s} dim={dim} n={n} prec={prec}", flush=True)
+        ref = O.GridSet(descs64[s])
+        ref[0].set_psi(psis[s])
+        ro = O.Propagator(ref, pdesc, [fields[s]]).run()
+        rb = b.records(s)
+        assert rb.shape[0] == ro.shape[0]
+        assert rel(b.psi(s), ref[0].psi()) < tol
+        assert np.allclose(rb[:, 2], ro[:, T.REC_NORM], rtol=1e-11 if prec == T.FP64 else 1e-5)
+        if prec == T.FP64:
+            assert np.allclose(rb[:, 3:], ro[:, T.REC_FLUX0:], rtol=1e-8,
+                               atol=1e-12 * max(np.abs(ro[:, T.REC_FLUX0:]).max(), 1e-300))
+    b.close()
+
+
 def test_cos18_edge_node_stays_finite(ctx):
     """cos^(1/8) with a width whose π/2·node/nCAP rounds past π/2 at the edge node (nCAP=13):
     the mask there is exactly 0 on both sides and the propagation stays finite."""
