@@ -294,7 +294,8 @@ template <int N, int ALT, typename T, bool PRE = true> struct Twiddles {
     return o;
   }
   static constexpr int kTotal = offset(PL::NST);
-  static constexpr bool kPreload = PRE && kTotal > 0 && kTotal <= 6;
+  // FP32 transforms read every power from the root table (below): nothing to preload
+  static constexpr bool kPreload = PRE && kTotal > 0 && kTotal <= 6 && sizeof(T) == 16;
   Rec r[kPreload ? kTotal : 1];
 
   __device__ __forceinline__ void load(const T* roots, int t) {
@@ -350,16 +351,25 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
     T x[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
-    if constexpr (NS > 1) {
+    if constexpr (NS > 1 && sizeof(T) == 8) {
+      // FP32: w^(r*k) is entry r*k*TWS (< N) of the root table, read as its {hi, lo} record and
+      // applied by the compensated product; no FP64 arithmetic and no power products
+      const float4* tab = reinterpret_cast<const float4*>(roots);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const float4 q = tab[r * k * TWS];
+        const float2 wh{q.x, DIR > 0 ? -q.y : q.y};
+        const float2 wl{q.z, DIR > 0 ? -q.w : q.w};
+        x[r] = cmul_comp(x[r], wh, wl);
+      }
+    }
+    if constexpr (NS > 1 && sizeof(T) == 16) {
       // w^k and w^4k (preloaded or read now) in FP64; the other powers by at most three FP64
-      // products. FP32 roots are {hi, lo} records taken to FP64, and every power is split back
-      // into hi + lo for the compensated product.
+      // products
       using Rec = typename TW::Rec;
       const Rec* tab = reinterpret_cast<const Rec*>(roots);
       auto as_root = [&](const Rec& q) {
-        double2 d;
-        if constexpr (sizeof(Rec) == 16 && sizeof(T) == 16) d = q;
-        else d = double2{(double)q.x + (double)q.z, (double)q.y + (double)q.w};
+        double2 d = q;
         if constexpr (DIR > 0) d.y = -d.y;
         return d;
       };
@@ -386,13 +396,7 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
           const double2 bb = r < 12 ? w8 : cmul(w8, w4);
           w = m == 0 ? bb : cmul(bb, m == 1 ? w1 : (m == 2 ? w2 : w3));
         }
-        if constexpr (sizeof(T) == 16) {
-          x[r] = cmul(x[r], w);
-        } else {
-          const float2 wh{(float)w.x, (float)w.y};
-          const float2 wl{(float)(w.x - (double)wh.x), (float)(w.y - (double)wh.y)};
-          x[r] = cmul_comp(x[r], wh, wl);
-        }
+        x[r] = cmul(x[r], w);
       }
     }
     dft<R, DIR>(x);
