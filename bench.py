@@ -166,26 +166,32 @@ def run_c4(steps, warmup):
     ctx = hip.Context(0)
     descs32, fields = c4_solves(T.FP32_MIXED)
     pd = c4_prop()
-    # autotune every size once for both precisions (single grids share the tuning table)
+    # autotune every size once per precision (single grids share the tuning table)
     for n in C4_SIDES:
-        for prec in (T.FP32_MIXED, T.FP64):
+        for prec in (T.FP32_MIXED, T.FP32_STATE, T.FP64):
             gs = hip.GridSet(ctx, c2_desc(prec, multigrid=0, n=n))
             gs[0].set_psi(ground_like_state(n))
             gs[0].autotune(5)
             gs.close()
-    progress("autotuned; building the batch")
-    b = hip.Batch(ctx, descs32, fields, pd)
-    for s, d in enumerate(descs32):
-        b.set_psi(s, ground_like_state(d.n[0]))
-    b.start()
-    b.run(warmup, use_graph=True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    b.run(steps, use_graph=True)
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
+    progress("autotuned; building the batches")
     pts = sum(int(d.n[0]) * int(d.n[1]) for d in descs32)
+
+    def timed_batch(prec):
+        descs, _ = c4_solves(prec)
+        b = hip.Batch(ctx, descs, fields, pd)
+        for s, d in enumerate(descs):
+            b.set_psi(s, ground_like_state(d.n[0]))
+        b.start()
+        b.run(warmup, use_graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        b.run(steps, use_graph=True)
+        ctx.synchronize()
+        return b, time.perf_counter() - t0
+
+    b, wall = timed_batch(T.FP32_MIXED)
     batched = pts * steps / wall
+    bs, wall_s = timed_batch(T.FP32_STATE)
     # every solve checked against the same solve advanced in FP64 for the same steps
     progress("checking each solve against an FP64 batch")
     descs64, _ = c4_solves(T.FP64)
@@ -195,20 +201,26 @@ def run_c4(steps, warmup):
     b64.start()
     b64.run(warmup + steps, use_graph=True)
     ctx.synchronize()
+
     # a batch state downloads only after its whole pulse; mid-run, every observed record row
     # (norm, fluxes) and the FP64 flux time integrals are compared at the same step
-    worst = dict(norm_rel_err=0.0, flux_integral_max_rel_err=0.0)
-    for s in range(len(descs32)):
-        r32, r64 = b.records(s), b64.records(s)
-        k = min(len(r32), len(r64))
-        worst["norm_rel_err"] = max(worst["norm_rel_err"],
-                                    float(np.max(np.abs(r32[:k, T.REC_NORM] / r64[:k, T.REC_NORM] - 1))))
-        f32, f64 = b.flux_integral(s), b64.flux_integral(s)
-        scale = np.abs(f64).max()
-        if scale > 0:
-            worst["flux_integral_max_rel_err"] = max(worst["flux_integral_max_rel_err"],
-                                                     float(np.abs(f32 - f64).max() / scale))
+    def worst_of(bb):
+        worst = dict(norm_rel_err=0.0, flux_integral_max_rel_err=0.0)
+        for s in range(len(descs32)):
+            r32, r64 = bb.records(s), b64.records(s)
+            k = min(len(r32), len(r64))
+            worst["norm_rel_err"] = max(worst["norm_rel_err"],
+                                        float(np.max(np.abs(r32[:k, T.REC_NORM] / r64[:k, T.REC_NORM] - 1))))
+            f32, f64 = bb.flux_integral(s), b64.flux_integral(s)
+            scale = np.abs(f64).max()
+            if scale > 0:
+                worst["flux_integral_max_rel_err"] = max(worst["flux_integral_max_rel_err"],
+                                                         float(np.abs(f32 - f64).max() / scale))
+        return dict(worst, steps=warmup + steps, solves=len(descs32))
+
+    worst, worst_s = worst_of(b), worst_of(bs)
     b64.close()
+    bs.close()
     b.close()
     # the plain FP64 step: one propagator per solve, per-size step time
     per_size = {}
@@ -231,7 +243,9 @@ def run_c4(steps, warmup):
     return dict(value=batched, ms_per_step=wall / steps * 1e3, pts=pts, plain=plain,
                 plain_ms_per_step=plain_s * 1e3, per_size_ms={k: round(v * 1e3, 4) for k, v in per_size.items()},
                 sides={int(n): int(sum(1 for d in descs32 if d.n[0] == n)) for n in C4_SIDES},
-                vs_fp64=dict(worst, steps=warmup + steps, solves=len(descs32)))
+                vs_fp64=worst,
+                fp32_state=dict(value=pts * steps / wall_s, ms_per_step=wall_s / steps * 1e3,
+                                speedup_vs_plain_fp64=round(pts * steps / wall_s / plain, 3), vs_fp64=worst_s))
 
 
 def run_c5(count, streams=4):
@@ -425,7 +439,8 @@ FP32_BAR = 1e-6
 
 
 def fp32_vs_fp64(m, r, nb):
-    """Errors of the FP32-mixed run `m` against the FP64 run `r` after the same steps."""
+    """Errors of an FP32-state run `m` (FP32-mixed or FP32 state with FP64 arithmetic) against
+    the FP64 run `r` after the same steps."""
     a, b = m["records"], r["records"]
     fa = a[:, T.REC_FLUX0:T.REC_FLUX0 + nb]
     fb = b[:, T.REC_FLUX0:T.REC_FLUX0 + nb]
@@ -495,7 +510,8 @@ def main():
                "plain_fp64": {"value": round(r["plain"], 1), "ms_per_step": round(r["plain_ms_per_step"], 4),
                               "per_size_ms_per_step": r["per_size_ms"]},
                "speedup_vs_plain_fp64": round(r["value"] / r["plain"], 3),
-               "vs_fp64_same_steps": r["vs_fp64"]}
+               "vs_fp64_same_steps": r["vs_fp64"],
+               "fp32_state": r["fp32_state"]}
         print(json.dumps(out))
         return
     if args.workload == "c2":
@@ -544,6 +560,14 @@ def main():
                              "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4),
                              "vs_fp64_same_steps": fp32_vs_fp64(m, r, nb)}
         del m
+        if args.workload == "c3":
+            # the FP32 state with FP64 arithmetic (multigrid sets take FP64 or FP32-mixed only)
+            m = run_device(c3_desc(T.FP32_STATE), pdesc, fields, args.steps, args.warmup, profile=False,
+                           keep_state=True)
+            out["fp32_state"] = {"value": round(m["m_main"] * m["done"] / m["wall"], 1),
+                                 "ms_per_step": round(m["wall"] / m["done"] * 1e3, 4),
+                                 "vs_fp64_same_steps": fp32_vs_fp64(m, r, nb)}
+            del m
     r["psi"] = None
     if args.workload == "c3" and not args.no_c4:
         c4 = run_c4(args.steps, args.warmup)
@@ -552,7 +576,8 @@ def main():
             "ms_per_step": round(c4["ms_per_step"], 4),
             "plain_fp64_value": round(c4["plain"], 1), "plain_fp64_ms_per_step": round(c4["plain_ms_per_step"], 4),
             "speedup_vs_plain_fp64": round(c4["value"] / c4["plain"], 3),
-            "vs_fp64_same_steps": c4["vs_fp64"]}
+            "vs_fp64_same_steps": c4["vs_fp64"],
+            "fp32_state": c4["fp32_state"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_c3(pdesc, fields) if args.workload == "c3" else \
             cpu_baseline_c2(desc, pdesc, fields)

@@ -43,8 +43,11 @@ enum { QSF_GAUGE_NONE = 0, QSF_GAUGE_VELOCITY = 1, QSF_GAUGE_LENGTH = 2 };
 /* Absorbers. CAP follows absorbers.h:16-60; COS18 is the cos^(1/8) edge mask, new here. */
 enum { QSF_ABS_NONE = 0, QSF_ABS_CAP = 1, QSF_ABS_COS18 = 2 };
 
-/* Arithmetic of the device step. */
-enum { QSF_FP64 = 0, QSF_FP32_MIXED = 1 };
+/* Arithmetic of the device step. FP32_MIXED: complex FP32 state and FP32 butterflies with
+ * compensated twiddles and constants, FP64 phase tables and reductions. FP32_STATE: the same
+ * FP32 state in HBM (half the bytes of FP64), every transform and phase product in FP64, one
+ * rounding per element at each store; single-region grids and batches (multigrid sets refuse it). */
+enum { QSF_FP64 = 0, QSF_FP32_MIXED = 1, QSF_FP32_STATE = 2 };
 
 /* Pulse shapes (field_p.h:47-84) and envelopes (field_p.h:88-194). */
 enum {
@@ -88,7 +91,7 @@ enum {
 /* One grid (CartesianGrid_ coords.h:14-139 + LocalGrid grid.h:63-123 + absorber + potential). */
 typedef struct qsf_grid_desc {
   int32_t dim;            /* 1..3                                                   */
-  int32_t precision;      /* QSF_FP64 | QSF_FP32_MIXED                              */
+  int32_t precision;      /* QSF_FP64 | QSF_FP32_MIXED | QSF_FP32_STATE             */
   int64_t n[3];           /* nodes per axis (unused axes: 1)                        */
   double dx[3];           /* spacing per axis                                       */
   int32_t potential;      /* QSF_POT_*                                              */

@@ -3,6 +3,7 @@
 // axis_pass_inst.hip; this file only dispatches on the runtime axis length.
 #include <hip/hip_runtime.h>
 
+#include "../../include/qsf_types.h"
 #include "axis_pass_impl.hpp"
 
 namespace qsf {
@@ -18,11 +19,16 @@ QSF_SIZES(QSF_EXTERN_D_ROW)
 QSF_SIZES(QSF_EXTERN_D_COL)
 QSF_SIZES(QSF_EXTERN_F_ROW)
 QSF_SIZES(QSF_EXTERN_F_COL)
+#define QSF_EXTERN_S32(ROW, NN) extern template int launch_size<double, ROW, NN, true>(int, const PassParams&, hipStream_t);
+#define QSF_EXTERN_S_ROW(NN) QSF_EXTERN_S32(true, NN)
+#define QSF_EXTERN_S_COL(NN) QSF_EXTERN_S32(false, NN)
+QSF_SIZES(QSF_EXTERN_S_ROW)
+QSF_SIZES(QSF_EXTERN_S_COL)
 
-template <typename Real, bool ROW>
+template <typename Real, bool ROW, bool S32 = false>
 static int dispatch(int64_t n, int v, const PassParams& p, hipStream_t s) {
   switch (n) {
-#define QSF_CASE(NN) case NN: return launch_size<Real, ROW, NN>(v, p, s);
+#define QSF_CASE(NN) case NN: return launch_size<Real, ROW, NN, S32>(v, p, s);
     QSF_SIZES(QSF_CASE)
 #undef QSF_CASE
     default: return -1;
@@ -55,7 +61,9 @@ int launch_axis_pass(int precision, const PassParams& p, hipStream_t s, int v) {
   const int64_t n = p.n[p.axis];
   const bool row = (p.inner == 1);
   if (v < 0 || v >= axis_variants(n)) v = 0;
-  if (precision == 0) return row ? dispatch<double, true>(n, v, p, s) : dispatch<double, false>(n, v, p, s);
+  if (precision == QSF_FP64) return row ? dispatch<double, true>(n, v, p, s) : dispatch<double, false>(n, v, p, s);
+  if (precision == QSF_FP32_STATE)
+    return row ? dispatch<double, true, true>(n, v, p, s) : dispatch<double, false, true>(n, v, p, s);
   return row ? dispatch<float, true>(n, v, p, s) : dispatch<float, false>(n, v, p, s);
 }
 
@@ -63,7 +71,9 @@ int64_t axis_pass_tiles(int precision, const PassParams& p, int v) {
   const int64_t n = p.n[p.axis];
   const bool row = (p.inner == 1);
   if (v < 0 || v >= axis_variants(n)) v = 0;
-  if (precision == 0) return row ? tiles<double, true>(n, v, p) : tiles<double, false>(n, v, p);
+  // FP64 arithmetic over an FP32 state tiles like the FP64 kernels
+  if (precision == QSF_FP64 || precision == QSF_FP32_STATE)
+    return row ? tiles<double, true>(n, v, p) : tiles<double, false>(n, v, p);
   return row ? tiles<float, true>(n, v, p) : tiles<float, false>(n, v, p);
 }
 
@@ -94,7 +104,7 @@ __global__ void gather_x_kernel(GatherXArgs a) {
 void launch_gather_x(int precision, const GatherXArgs& a, hipStream_t s) {
   if (a.nvals <= 0) return;
   const int blocks = (int)((a.nvals + 255) / 256 < 65536 ? (a.nvals + 255) / 256 : 65536);
-  if (precision == 0) hipLaunchKernelGGL(gather_x_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
+  if (precision == QSF_FP64) hipLaunchKernelGGL(gather_x_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(gather_x_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
 }
 

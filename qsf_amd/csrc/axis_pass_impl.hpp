@@ -24,6 +24,18 @@ template <typename Real> struct Tr {
   using T = typename cvec<Real>::T;
 };
 
+// Stored state type of a pass: the arithmetic type's complex vector, or (S32, FP64 arithmetic
+// over an FP32 state) float2 — read and widened at the load, rounded once at each store.
+template <typename Real, bool S32> using StateOf = std::conditional_t<S32, float2, typename Tr<Real>::T>;
+template <typename S, typename T> __device__ __forceinline__ S to_state(T v) {
+  if constexpr (std::is_same_v<S, T>) return v;
+  else return S{(float)v.x, (float)v.y};
+}
+template <typename T, typename S> __device__ __forceinline__ T from_state(S v) {
+  if constexpr (std::is_same_v<S, T>) return v;
+  else return T{(double)v.x, (double)v.y};
+}
+
 // Row-mode exchange layout. Stockham stage 0 writes element j*R + r from lane j (stride R);
 // unpadded, 16 lanes of one write pass then hit two 4-byte-bank groups (FP64, R = 8: 128 B
 // apart) — an 8-way conflict. One pad element per 256 B of line (16 complex FP64 or 32 complex
@@ -283,12 +295,13 @@ __device__ __forceinline__ int64_t tile_at(const PassParams& p, int64_t k, int64
 
 // One tile of the pass on the state `v` already loaded: everything after the load, through the
 // store. An observation leaves each wave's sums in the reduction area of LDS.
-template <typename Real, int N, bool ROW, int V, int FEAT>
+template <typename Real, int N, bool ROW, int V, int FEAT, bool S32>
 __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, ROW, Geo<Real, N, ROW, V>::W>& g,
                                           typename Tr<Real>::T (&v)[PlanOf<N, var_alt(V)>::E], int tid, int t, int w,
-                                          unsigned char* smem_raw, typename Tr<Real>::T* dst, const void* pf0,
+                                          unsigned char* smem_raw, StateOf<Real, S32>* dst, const void* pf0,
                                           const void* pf1, const void* pf2, void* gather_out) {
   using T = typename Tr<Real>::T;
+  using S = StateOf<Real, S32>;
   using G = Geo<Real, N, ROW, V>;
   constexpr int ALT = var_alt(V);
   using PL = PlanOf<N, ALT>;
@@ -439,7 +452,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
           ep += a2 * pot_value(ob, co, D);
         }
         if (ka || kc != 0.0) ek += a2 * (kc + (ka ? ka[ia] : 0.0));
-        if (ob.copy_out) reinterpret_cast<T*>(ob.copy_out)[base + (int64_t)ia * step] = v[q];
+        if (ob.copy_out) reinterpret_cast<S*>(ob.copy_out)[base + (int64_t)ia * step] = to_state<S>(v[q]);
       }
       if (gline) glds[w * G::gather_stride + ia] = v[q];
       if (ROW && p.er_out && valid) {
@@ -447,11 +460,11 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
         const int64_t nc = p.nCAP, n0 = p.n[0], n1 = N;
         if (o < nc || o >= n0 - nc) {
           const int64_t e = o < nc ? o : o - (n0 - 2 * nc);
-          reinterpret_cast<T*>(p.er_out)[e * n1 + ia] = v[q];
+          reinterpret_cast<S*>(p.er_out)[e * n1 + ia] = to_state<S>(v[q]);
         }
         if (ia < nc || ia >= n1 - nc) {
           const int64_t e = ia < nc ? ia : ia - (n1 - 2 * nc);
-          reinterpret_cast<T*>(p.ec_out)[o * 2 * nc + e] = v[q];
+          reinterpret_cast<S*>(p.ec_out)[o * 2 * nc + e] = to_state<S>(v[q]);
         }
       }
       if (msk) {
@@ -481,12 +494,12 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
       if (gather) {
         exchange_sync<G::WAVE_LINES>();
         if (valid) {
-          T* go = reinterpret_cast<T*>(gather_out);
+          S* go = reinterpret_cast<S*>(gather_out);
           for (int k = gk0 + t; k < gk1; k += TT) {
             const int pos = ob.gather_pos[k];
             T val = glds[w * G::gather_stride + pos];
             if (compact && doA) val = capply(val, xa.at(pos));   // the same product as a full pass
-            go[k] = val;
+            go[k] = to_state<S>(val);
           }
         }
         exchange_sync<G::WAVE_LINES>();   // the line's LDS is written again by the next exchange
@@ -517,7 +530,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
 
   if (valid && !(p.flags & F_NOSTORE)) {
 #pragma unroll
-    for (int q = 0; q < E; ++q) dst[base + (int64_t)(t + TT * q) * step] = v[q];
+    for (int q = 0; q < E; ++q) dst[base + (int64_t)(t + TT * q) * step] = to_state<S>(v[q]);
   }
 }
 
@@ -538,7 +551,7 @@ constexpr int pass_waves() {
   return 1;
 }
 
-template <typename Real, int N, bool ROW, int V, int FEAT>
+template <typename Real, int N, bool ROW, int V, int FEAT, bool S32>
 __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
                                   (pass_waves<PlanOf<N, var_alt(V)>::E, FEAT, ROW && PlanOf<N, var_alt(V)>::T % 64 == 0>()))
     axis_pass_kernel(PassParams p) {
@@ -556,8 +569,9 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
   // lives in scalar registers
   const int w = ROW ? (TT % 64 == 0 ? __builtin_amdgcn_readfirstlane(tid / TT) : tid / TT) : (tid % W);
 
-  const T* src = reinterpret_cast<const T*>(p.src);
-  T* dst = reinterpret_cast<T*>(p.dst);
+  using S = StateOf<Real, S32>;
+  const S* src = reinterpret_cast<const S*>(p.src);
+  S* dst = reinterpret_cast<S*>(p.dst);
   const void* pf0 = p.pf[0];
   const void* pf1 = p.pf[1];
   const void* pf2 = p.pf[2];
@@ -567,8 +581,8 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
     // batched launch: this block's solve (whole block leaves together, before any barrier)
     const BatchEntry& be = p.batch[blockIdx.y];
     if (*p.step_ctr >= be.total_steps) return;
-    src = reinterpret_cast<const T*>(be.psi);
-    dst = reinterpret_cast<T*>(be.psi);
+    src = reinterpret_cast<const S*>(be.psi);
+    dst = reinterpret_cast<S*>(be.psi);
     pf0 = be.pf[0];
     pf1 = be.pf[1];
     pf2 = be.pf[2];
@@ -585,8 +599,9 @@ __global__ void __launch_bounds__((Geo<Real, N, ROW, V>::THREADS),
   g.at(p, tile_at<W, ROW>(p, k0, ntiles), w);
   T v[E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = g.valid ? src[g.base + (int64_t)(t + TT * q) * g.step] : T{Real(0), Real(0)};
-  pass_tile<Real, N, ROW, V, FEAT>(p, g, v, tid, t, w, smem_raw, dst, pf0, pf1, pf2, gather_out);
+  for (int q = 0; q < E; ++q)
+    v[q] = g.valid ? from_state<T>(src[g.base + (int64_t)(t + TT * q) * g.step]) : T{Real(0), Real(0)};
+  pass_tile<Real, N, ROW, V, FEAT, S32>(p, g, v, tid, t, w, smem_raw, dst, pf0, pf1, pf2, gather_out);
   if (FEAT != FEAT_LEAN && (p.flags & F_OBS)) {
     // the block's sums: its waves' sums in wave order
     using G = Geo<Real, N, ROW, V>;
@@ -609,7 +624,7 @@ inline int64_t launch_blocks(const PassParams& p) {
   return ROW ? (p.outer + G::W - 1) / G::W : p.outer * ((p.inner + G::W - 1) / G::W);
 }
 
-template <typename Real, int N, bool ROW, int V, int FEAT>
+template <typename Real, int N, bool ROW, int V, int FEAT, bool S32>
 inline int launch_feat(const PassParams& p, hipStream_t s) {
   using G = Geo<Real, N, ROW, V>;
   // LDS per launch: the transform exchange; for a flux gather the complex lines (inside the
@@ -619,7 +634,7 @@ inline int launch_feat(const PassParams& p, hipStream_t s) {
   static_assert(most <= 163840, "a block's LDS exceeds the 160 KiB of a CU");
   // once per kernel (a function-local static initialises once, also under concurrent callers)
   static const bool attr_set = [] {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V, FEAT>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&axis_pass_kernel<Real, N, ROW, V, FEAT, S32>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
     return true;
   }();
@@ -628,19 +643,19 @@ inline int launch_feat(const PassParams& p, hipStream_t s) {
   if (FEAT != FEAT_LEAN && (p.flags & F_OBS)) lds = most;
   const unsigned ny = p.batch ? (unsigned)p.nbatch : 1u;
   const int64_t blocks = launch_blocks<Real, N, ROW, V>(p);
-  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V, FEAT>), dim3((unsigned)blocks, ny), dim3(G::THREADS), lds, s,
+  hipLaunchKernelGGL((axis_pass_kernel<Real, N, ROW, V, FEAT, S32>), dim3((unsigned)blocks, ny), dim3(G::THREADS), lds, s,
                      p);
   return (int)blocks;
 }
 
-template <typename Real, int N, bool ROW, int V>
+template <typename Real, int N, bool ROW, int V, bool S32>
 inline int launch_one(const PassParams& p, hipStream_t s) {
   const int f = feat_of(p, ROW);
-  if (f == FEAT_LEAN) return launch_feat<Real, N, ROW, V, FEAT_LEAN>(p, s);
+  if (f == FEAT_LEAN) return launch_feat<Real, N, ROW, V, FEAT_LEAN, S32>(p, s);
   if constexpr (ROW) {
-    if (f == FEAT_X) return launch_feat<Real, N, ROW, V, FEAT_X>(p, s);
+    if (f == FEAT_X) return launch_feat<Real, N, ROW, V, FEAT_X, S32>(p, s);
   }
-  return launch_feat<Real, N, ROW, V, FEAT_FULL>(p, s);
+  return launch_feat<Real, N, ROW, V, FEAT_FULL, S32>(p, s);
 }
 
 // Blocks of a launch, which is also the number of per-block observation sums it writes.
@@ -657,20 +672,21 @@ inline bool has_variants(int64_t n) { return tuned_length(n); }
 
 // Launcher and tile count of one axis length (all its variants). Instantiated explicitly per
 // (precision, mode, length group) in axis_pass_inst.hip, so the heavy kernels compile in
-// parallel translation units; axis_pass.hip dispatches on the runtime length.
-template <typename Real, bool ROW, int N>
+// parallel translation units; axis_pass.hip dispatches on the runtime length. S32: FP64
+// arithmetic over an FP32 state (same tiles as the FP64 kernels).
+template <typename Real, bool ROW, int N, bool S32 = false>
 int launch_size(int v, const PassParams& p, hipStream_t s) {
   if constexpr (tuned_length(N)) {
     switch (v) {
-      case 1: return launch_one<Real, N, ROW, 1>(p, s);
-      case 2: return launch_one<Real, N, ROW, 2>(p, s);
-      case 3: return launch_one<Real, N, ROW, 3>(p, s);
-      case 4: return launch_one<Real, N, ROW, 4>(p, s);
-      case 5: return launch_one<Real, N, ROW, 5>(p, s);
+      case 1: return launch_one<Real, N, ROW, 1, S32>(p, s);
+      case 2: return launch_one<Real, N, ROW, 2, S32>(p, s);
+      case 3: return launch_one<Real, N, ROW, 3, S32>(p, s);
+      case 4: return launch_one<Real, N, ROW, 4, S32>(p, s);
+      case 5: return launch_one<Real, N, ROW, 5, S32>(p, s);
       default: break;
     }
   }
-  return launch_one<Real, N, ROW, 0>(p, s);
+  return launch_one<Real, N, ROW, 0, S32>(p, s);
 }
 
 template <typename Real, bool ROW, int N>

@@ -1,6 +1,7 @@
 // axis_pass_inst.hip — explicit instantiations of the fused axis pass for one precision, one
 // mode and one group of axis lengths. The Makefile compiles this file once per
-// (QSF_PREC: 0 = double, 1 = float; QSF_ROWMODE: 1 = row, 0 = column; QSF_GROUP: 0..11), so the
+// (QSF_PREC: 0 = double, 1 = float, 2 = double arithmetic over a float state; QSF_ROWMODE: 1 =
+// row, 0 = column; QSF_GROUP: 0..11), so the
 // kernels of the long transforms build in parallel translation units.
 #include "axis_pass_impl.hpp"
 
@@ -8,10 +9,10 @@
 #error "QSF_PREC, QSF_ROWMODE and QSF_GROUP must be defined"
 #endif
 
-#if QSF_PREC == 0
-using QsfReal = double;
-#else
+#if QSF_PREC == 1
 using QsfReal = float;
+#else
+using QsfReal = double;
 #endif
 
 #if QSF_GROUP == 0
@@ -43,9 +44,15 @@ using QsfReal = float;
 #endif
 
 namespace qsf {
+#if QSF_PREC == 2
+// the tile counts are the FP64 kernels' (instantiated with QSF_PREC = 0)
+#define QSF_INST(NN) \
+  template int launch_size<double, (QSF_ROWMODE != 0), NN, true>(int, const PassParams&, hipStream_t);
+#else
 #define QSF_INST(NN)                                                                            \
   template int launch_size<QsfReal, (QSF_ROWMODE != 0), NN>(int, const PassParams&, hipStream_t); \
   template int64_t tiles_size<QsfReal, (QSF_ROWMODE != 0), NN>(int, const PassParams&);
+#endif
 QSF_GROUP_SIZES(QSF_INST)
 #undef QSF_INST
 }  // namespace qsf

@@ -204,7 +204,7 @@ struct Grid {
     D = d.dim;
     QSF_REQUIRE(D >= 1 && D <= 3, QSF_ERR_ARG, "dim must be 1..3");
     prec = d.precision;
-    QSF_REQUIRE(prec == QSF_FP64 || prec == QSF_FP32_MIXED, QSF_ERR_ARG, "bad precision");
+    QSF_REQUIRE(prec == QSF_FP64 || prec == QSF_FP32_MIXED || prec == QSF_FP32_STATE, QSF_ERR_ARG, "bad precision");
     cb = prec == QSF_FP64 ? 16 : 8;
     m = 1;
     for (int i = 0; i < 3; ++i) {
@@ -264,8 +264,9 @@ struct Grid {
     auto it = roots.find(len);
     if (it != roots.end()) return it->second->p;
     auto b = std::make_unique<DBuf>();
-    b->ensure(len * cb * (prec == QSF_FP64 ? 1 : 2));   // FP32: {hi, lo} record per root
-    launch_roots(prec, b->p, len, st());
+    // 16 bytes per root: the FP64 root (FP64 and FP32-state arithmetic), or the FP32 {hi, lo}
+    b->ensure(len * 16);
+    launch_roots(prec == QSF_FP32_MIXED ? QSF_FP32_MIXED : QSF_FP64, b->p, len, st());
     const void* p = b->p;
     roots[len] = std::move(b);
     return p;
@@ -537,7 +538,8 @@ struct Grid {
     if (ctx->prof) {
       QSF_HIP(hipEventRecord(eb, st()));
       const bool row = p.inner == 1;
-      std::string key = std::string("pass_") + (prec == QSF_FP64 ? "f64" : "f32") + (row ? "_row_n" : "_col_n") +
+      std::string key = std::string("pass_") + (prec == QSF_FP64 ? "f64" : (prec == QSF_FP32_STATE ? "s32" : "f32")) +
+                        (row ? "_row_n" : "_col_n") +
                         std::to_string(n[p.axis]) + "_v" + std::to_string(var);
       const double bytes = (double)m * cb * ((p.flags & F_NOSTORE) ? 1.0 : 2.0);
       ctx->launches.push_back({key, ea, eb, bytes});
@@ -985,6 +987,8 @@ struct Set {
       return;
     }
     QSF_REQUIRE(D == 2 || D == 3, QSF_ERR_UNSUPPORTED, "multigrid regions need 2 or 3 dimensions");
+    QSF_REQUIRE(d.precision != QSF_FP32_STATE, QSF_ERR_UNSUPPORTED,
+                "multigrid regions run in FP64 or FP32-mixed arithmetic");
     if (D == 3) {
       // the transfer boxes (multigrid.h:188, 297-309) tile axes 1 and 2 in steps of nCAP and
       // are counted on axis 0: a cube whose side nCAP divides

@@ -13,7 +13,7 @@ AXIS_NO, AXIS_X, AXIS_Y, AXIS_Z, AXIS_XY, AXIS_XZ, AXIS_YZ, AXIS_XYZ = 0, 1, 2, 
 POT_NONE, POT_COULOMB, POT_EBERLY, POT_ECKHARDT_SACHA = 0, 1, 2, 3
 GAUGE_NONE, GAUGE_VELOCITY, GAUGE_LENGTH = 0, 1, 2
 ABS_NONE, ABS_CAP, ABS_COS18 = 0, 1, 2
-FP64, FP32_MIXED = 0, 1
+FP64, FP32_MIXED, FP32_STATE = 0, 1, 2
 PULSE_EMPTY, PULSE_SIN, PULSE_CONST, PULSE_CHEMPHYS = 0, 1, 2, 3
 ENV_NONE, ENV_SIN2, ENV_CHEMPHYS, ENV_RAMP, ENV_SIN, ENV_GAUSS = 0, 1, 2, 3, 4, 5
 (BORDER_BOX, BORDER_S2CAP, BORDER_D2CAP, BORDER_N2S, BORDER_N2D, BORDER_N2D_SYM,
