@@ -248,11 +248,12 @@ def run_c4(steps, warmup):
                                 speedup_vs_plain_fp64=round(pts * steps / wall_s / plain, 3), vs_fp64=worst_s))
 
 
-def run_c5(count, streams=4):
+def run_c5(count, streams=4, precision=T.FP32_MIXED):
     """The serving loop on a seeded Poisson stream (mean 4 requests/s) of mixed solves: 60% 1D
-    N=4096, 35% 2D sides 512-2048, 5% 3D 256^3, FP32-mixed, every 10th with an FP64 twin."""
+    N=4096, 35% 2D sides 512-2048, 5% 3D 256^3, FP32-mixed (or FP32 state with FP64
+    arithmetic), every 10th with an FP64 twin."""
     from qsf_amd import hip, serve
-    reqs = serve.poisson_stream(count, rate=4.0, seed=5)
+    reqs = serve.poisson_stream(count, rate=4.0, seed=5, precision=precision)
     # one autotuning pass per distinct grid shape and precision; every stream applies its table
     ctx = hip.Context(0)
     seen = set()
@@ -462,6 +463,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="c3: leave out the batched 2D comparison")
     ap.add_argument("--requests", type=int, default=40, help="c5: number of requests in the stream")
+    ap.add_argument("--precision", choices=["fp32_mixed", "fp32_state"], default="fp32_mixed",
+                    help="c5: arithmetic of the served requests")
     ap.add_argument("--no-variants", action="store_true", help="leave out the FP32-mixed run")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -484,11 +487,13 @@ def main():
         print(json.dumps(out))
         return
     if args.workload == "c5":
-        r = run_c5(args.requests)
+        prec = T.FP32_STATE if args.precision == "fp32_state" else T.FP32_MIXED
+        r = run_c5(args.requests, precision=prec)
         out = {"metric": "grid-point-steps/s", "value": round(r["throughput_grid_point_steps_per_s"], 1),
                "unit": "grid-point-steps/s", "n_gpus": 1, "steps": None, "warmup": 0,
                "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-               "dtype": "c64 (f32, f64 phases and reductions); FP64 twins for verification",
+               "dtype": ("c64 state, f64 arithmetic" if prec == T.FP32_STATE else
+                         "c64 (f32, f64 phases and reductions)") + "; FP64 twins for verification",
                "data": "synthetic",
                "config": {"workload": f"serving: {args.requests} requests, Poisson 4/s (seed 5), mix 60% 1D "
                                       "N=4096 / 35% 2D 512-2048 / 5% 3D 256^3, full pulses, continuous "

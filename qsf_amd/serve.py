@@ -142,7 +142,7 @@ def bound_state(desc):
 
 
 def poisson_stream(count, rate=4.0, seed=5, mix=(0.60, 0.35, 0.05), verify_every=10, **shape_kw):
-    """Seeded Poisson arrivals of mixed requests; every `verify_every`-th FP32-mixed request gets
+    """Seeded Poisson arrivals of mixed requests; every `verify_every`-th FP32 request gets
     an FP64 twin (same arrival) whose result checks it. The kinds hold the mix exactly (rounded
     counts in a seeded random order), so a short stream still carries its share of 3D solves."""
     rng = np.random.default_rng(seed)
