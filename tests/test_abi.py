@@ -43,3 +43,17 @@ def test_types_mirror_header_sizes():
     assert ctypes.sizeof(T.GridDesc) == 120
     assert ctypes.sizeof(T.FieldDesc) == 64
     assert ctypes.sizeof(T.PropDesc) == 176
+
+
+def test_enum_values_mirror_header():
+    # every `QSF_NAME = value` enumerator of qsf_types.h has the same value in qsf_types.py
+    from qsf_amd import qsf_types as T
+    src = open(os.path.join(ROOT, "include/qsf_types.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    found = 0
+    for body in re.findall(r"enum\s*\{(.*?)\}", src, flags=re.S):
+        for name, value, shift in re.findall(r"QSF_(\w+)\s*=\s*(-?\d+)(?:\s*<<\s*(\d+))?", body):
+            if hasattr(T, name):
+                assert getattr(T, name) == int(value) << int(shift or 0), name
+                found += 1
+    assert found >= 3 and T.FP32_STATE == 2
