@@ -317,7 +317,7 @@ __device__ __forceinline__ void pass_tile(const PassParams& p, const TileGeo<N, 
 
   // this thread's twiddles for every transform of the pass, and the flux gather's row range,
   // read with the state
-  Twiddles<N, ALT, T, FEAT == FEAT_LEAN> tw;
+  Twiddles<N, ALT, T, FEAT == FEAT_LEAN, !ROW> tw;   // FP32 column tiles read twiddles directly
   if (p.flags & (F_INV1 | F_FWD | F_INV2)) tw.load(roots, t);
   int gk0 = 0, gk1 = 0;
   if (FEAT != FEAT_LEAN && ROW && (p.flags & F_OBS) && p.obs.gather_rowptr && valid) {

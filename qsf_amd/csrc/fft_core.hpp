@@ -278,11 +278,17 @@ template <int N, int ALT = 0> struct PlanOf {
 // exchange barrier; forward and inverse transforms of the pass share them (conjugated). Larger
 // sets (the 24-element plans of the 3*2^k lengths), and kernels that keep PRE = false because
 // their registers are spoken for, read them stage by stage.
+// DIRECT32 (FP32 column tiles): every power w^(r k) is read as its own {hi, lo} record at the
+// stage, with no FP64 products and nothing preloaded. The FP64 power products held the 16-element
+// FP32 column kernels at 160 VGPRs (two waves per SIMD) and the direct reads bring them to 122;
+// FP32 rows keep the preloaded form, whose per-stage reads would wait behind block barriers
+// (measured: 2048-point rows 36 -> 46 us with direct reads, batched 2048-point columns
+// 797 -> 593 us).
 template <typename T> struct RootRec;
 template <> struct RootRec<double2> { using type = double2; };   // the FP64 root
 template <> struct RootRec<float2> { using type = float4; };     // FP32 {hi, lo} of the root
 
-template <int N, int ALT, typename T, bool PRE = true> struct Twiddles {
+template <int N, int ALT, typename T, bool PRE = true, bool DIRECT32 = false> struct Twiddles {
   using PL = PlanOf<N, ALT>;
   using Rec = typename RootRec<T>::type;
   static constexpr int slots(int S) {
@@ -294,8 +300,8 @@ template <int N, int ALT, typename T, bool PRE = true> struct Twiddles {
     return o;
   }
   static constexpr int kTotal = offset(PL::NST);
-  // FP32 transforms read every power from the root table (below): nothing to preload
-  static constexpr bool kPreload = PRE && kTotal > 0 && kTotal <= 6 && sizeof(T) == 16;
+  static constexpr bool kDirect = DIRECT32 && sizeof(T) == 8;
+  static constexpr bool kPreload = PRE && kTotal > 0 && kTotal <= 6 && !kDirect;
   Rec r[kPreload ? kTotal : 1];
 
   __device__ __forceinline__ void load(const T* roots, int t) {
@@ -332,11 +338,11 @@ template <typename T, typename Addr> constexpr bool kSplitFor = kSplitExchange<T
 // thread's transform through addr(p), in units of one exchanged value (kExchangeBytes).
 // roots[j] = exp(-2πi j/N) (N entries; in FP32 each entry is a pair {hi, lo}: roots[2j] the
 // rounded root, roots[2j + 1] its rounding remainder, read as one 16-byte record).
-template <int N, int ALT, int DIR, int S, typename T, typename Addr, bool PRE>
+template <int N, int ALT, int DIR, int S, typename T, typename Addr, bool PRE, bool D32>
 __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
-                                          const Twiddles<N, ALT, T, PRE>& tw) {
+                                          const Twiddles<N, ALT, T, PRE, D32>& tw) {
   using PL = PlanOf<N, ALT>;
-  using TW = Twiddles<N, ALT, T, PRE>;
+  using TW = Twiddles<N, ALT, T, PRE, D32>;
   constexpr int E = PL::E, TT = PL::T;
   constexpr int R = PL::template R<S>;
   constexpr int NS = PL::template NS<S>();
@@ -351,8 +357,8 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
     T x[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) x[r] = v[b + NB * r];
-    if constexpr (NS > 1 && sizeof(T) == 8) {
-      // FP32: w^(r*k) is entry r*k*TWS (< N) of the root table, read as its {hi, lo} record and
+    if constexpr (NS > 1 && TW::kDirect) {
+      // w^(r*k) is entry r*k*TWS (< N) of the root table, read as its {hi, lo} record and
       // applied by the compensated product; no FP64 arithmetic and no power products
       const float4* tab = reinterpret_cast<const float4*>(roots);
 #pragma unroll
@@ -363,13 +369,16 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
         x[r] = cmul_comp(x[r], wh, wl);
       }
     }
-    if constexpr (NS > 1 && sizeof(T) == 16) {
+    if constexpr (NS > 1 && !TW::kDirect) {
       // w^k and w^4k (preloaded or read now) in FP64; the other powers by at most three FP64
-      // products
+      // products. FP32 roots are {hi, lo} records taken to FP64, and every power is split back
+      // into hi + lo for the compensated product.
       using Rec = typename TW::Rec;
       const Rec* tab = reinterpret_cast<const Rec*>(roots);
       auto as_root = [&](const Rec& q) {
-        double2 d = q;
+        double2 d;
+        if constexpr (sizeof(T) == 16) d = q;
+        else d = double2{(double)q.x + (double)q.z, (double)q.y + (double)q.w};
         if constexpr (DIR > 0) d.y = -d.y;
         return d;
       };
@@ -396,7 +405,13 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
           const double2 bb = r < 12 ? w8 : cmul(w8, w4);
           w = m == 0 ? bb : cmul(bb, m == 1 ? w1 : (m == 2 ? w2 : w3));
         }
-        x[r] = cmul(x[r], w);
+        if constexpr (sizeof(T) == 16) {
+          x[r] = cmul(x[r], w);
+        } else {
+          const float2 wh{(float)w.x, (float)w.y};
+          const float2 wl{(float)(w.x - (double)wh.x), (float)(w.y - (double)wh.y)};
+          x[r] = cmul_comp(x[r], wh, wl);
+        }
       }
     }
     dft<R, DIR>(x);
@@ -449,9 +464,9 @@ __device__ __forceinline__ void fft_stage(T* v, int t, T* lds, const Addr& addr,
   }
 }
 
-template <int N, int ALT, int DIR, int S, typename T, typename Addr, bool PRE>
+template <int N, int ALT, int DIR, int S, typename T, typename Addr, bool PRE, bool D32>
 __device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
-                                           const Twiddles<N, ALT, T, PRE>& tw) {
+                                           const Twiddles<N, ALT, T, PRE, D32>& tw) {
   if constexpr (S < PlanOf<N, ALT>::NST) {
     fft_stage<N, ALT, DIR, S>(v, t, lds, addr, roots, tw);
     fft_stages<N, ALT, DIR, S + 1>(v, t, lds, addr, roots, tw);
@@ -460,9 +475,9 @@ __device__ __forceinline__ void fft_stages(T* v, int t, T* lds, const Addr& addr
 
 // `roots` is the root table of length N (FP64 roots, or FP32 {hi, lo} records); `tw` holds this
 // thread's twiddles when the plan preloads them (Twiddles::load at the start of the pass).
-template <int N, int ALT, int DIR, typename T, typename Addr, bool PRE>
+template <int N, int ALT, int DIR, typename T, typename Addr, bool PRE, bool D32>
 __device__ __forceinline__ void fft(T* v, int t, T* lds, const Addr& addr, const T* __restrict__ roots,
-                                    const Twiddles<N, ALT, T, PRE>& tw) {
+                                    const Twiddles<N, ALT, T, PRE, D32>& tw) {
   fft_stages<N, ALT, DIR, 0>(v, t, lds, addr, roots, tw);
 }
 
