@@ -680,28 +680,36 @@ __global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, con
   const double* C = a.coupling + 3 * (k - 1);   // A after the step's P substep
   const T* g = reinterpret_cast<const T*>(a.gathered);
   const int D = geo.D, nb = geo.nb, stride = 1 + 2 * D;
-  for (int f = 0; f < nb; ++f) {
-    double s = 0.0;
-    for (int64_t pidx = threadIdx.x; pidx < geo.npts; pidx += 256) {
-      const int32_t* q = geo.pts + pidx * stride;
-      const T c = g[q[0]];
-      const double cr = c.x, ci = c.y, nrm = cr * cr + ci * ci;
+  // one sweep over the points: each point's stencil is read once and its current added to every
+  // family's sum (each family keeps its own per-thread order over the points)
+  double fam[16];
+  for (int f = 0; f < nb; ++f) fam[f] = 0.0;
+  for (int64_t pidx = threadIdx.x; pidx < geo.npts; pidx += 256) {
+    const int32_t* q = geo.pts + pidx * stride;
+    const T c = g[q[0]];
+    const double cr = c.x, ci = c.y, nrm = cr * cr + ci * ci;
+    double jd[3];
+    for (int d = 0; d < D; ++d) {
+      jd[d] = 0.0;
+      const int32_t sp = q[1 + d], sm = q[1 + D + d];
+      if (sp >= 0 && sm >= 0) {
+        const double dr = (double)g[sp].x - (double)g[sm].x, di = (double)g[sp].y - (double)g[sm].y;
+        jd[d] = geo.inv_2dx[d] * (cr * di - ci * dr);
+      }
+      jd[d] += C[d] * nrm;
+    }
+    for (int f = 0; f < nb; ++f) {
       double j = 0.0;
       for (int d = 0; d < D; ++d) {
         const double bv = geo.bvec[(pidx * nb + f) * D + d];
         if (bv == 0.0) continue;
-        double jd = 0.0;
-        const int32_t sp = q[1 + d], sm = q[1 + D + d];
-        if (sp >= 0 && sm >= 0) {
-          const double dr = (double)g[sp].x - (double)g[sm].x, di = (double)g[sp].y - (double)g[sm].y;
-          jd = geo.inv_2dx[d] * (cr * di - ci * dr);
-        }
-        jd += C[d] * nrm;
-        j += jd * bv;
+        j += jd[d] * bv;
       }
-      s += j;
+      fam[f] += j;
     }
-    red[threadIdx.x] = s;
+  }
+  for (int f = 0; f < nb; ++f) {
+    red[threadIdx.x] = fam[f];
     block_sync();
     for (int t = 128; t > 0; t >>= 1) {
       if ((int)threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
