@@ -658,20 +658,24 @@ void launch_batch_ptables(int precision, const BatchTableArgs* dev_args, int nso
 }
 
 // One block per solve: norm from the pass partials and the flux families from the gathered
-// boundary points (same fixed-order sums as the single-solve path).
+// boundary points, each a fixed-order sum (thread-strided, then a tree). The blocks are few (one
+// per solve of a size group) and the point loop is a chain of dependent loads, so the block is
+// as wide as it goes.
+constexpr int kObsThreads = 1024;
 template <typename Real>
-__global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, const int64_t* step_ctr) {
+__global__ void __launch_bounds__(kObsThreads)
+    batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, const int64_t* step_ctr) {
   using T = typename cvec<Real>::T;
   const BatchObsArgs& a = args[blockIdx.x];
   const int64_t k = *step_ctr;            // steps completed at the observed boundary
   if (k > a.total_steps || k < 1) return;
-  __shared__ double red[256];
+  __shared__ double red[kObsThreads];
   double* row = a.rec + k * (1 + geo.nb);
   double acc = 0.0;
-  for (int64_t b = threadIdx.x; b < a.tiles; b += 256) acc += a.partials[3 * b];
+  for (int64_t b = threadIdx.x; b < a.tiles; b += kObsThreads) acc += a.partials[3 * b];
   red[threadIdx.x] = acc;
   block_sync();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kObsThreads / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     block_sync();
   }
@@ -684,7 +688,7 @@ __global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, con
   // family's sum (each family keeps its own per-thread order over the points)
   double fam[16];
   for (int f = 0; f < nb; ++f) fam[f] = 0.0;
-  for (int64_t pidx = threadIdx.x; pidx < geo.npts; pidx += 256) {
+  for (int64_t pidx = threadIdx.x; pidx < geo.npts; pidx += kObsThreads) {
     const int32_t* q = geo.pts + pidx * stride;
     const T c = g[q[0]];
     const double cr = c.x, ci = c.y, nrm = cr * cr + ci * ci;
@@ -711,7 +715,7 @@ __global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, con
   for (int f = 0; f < nb; ++f) {
     red[threadIdx.x] = fam[f];
     block_sync();
-    for (int t = 128; t > 0; t >>= 1) {
+    for (int t = kObsThreads / 2; t > 0; t >>= 1) {
       if ((int)threadIdx.x < t) red[threadIdx.x] += red[threadIdx.x + t];
       block_sync();
     }
@@ -726,9 +730,9 @@ __global__ void batch_obs_kernel(const BatchObsArgs* args, BatchFluxGeo geo, con
 void launch_batch_obs(const BatchObsArgs* dev_args, int nsolves, const BatchFluxGeo& geo,
                       const int64_t* step_ctr, hipStream_t s) {
   if (geo.precision == QSF_FP64)
-    hipLaunchKernelGGL(batch_obs_kernel<double>, dim3(nsolves), dim3(256), 0, s, dev_args, geo, step_ctr);
+    hipLaunchKernelGGL(batch_obs_kernel<double>, dim3(nsolves), dim3(kObsThreads), 0, s, dev_args, geo, step_ctr);
   else
-    hipLaunchKernelGGL(batch_obs_kernel<float>, dim3(nsolves), dim3(256), 0, s, dev_args, geo, step_ctr);
+    hipLaunchKernelGGL(batch_obs_kernel<float>, dim3(nsolves), dim3(kObsThreads), 0, s, dev_args, geo, step_ctr);
 }
 
 __global__ void step_increment_kernel(int64_t* c) {
