@@ -1690,8 +1690,21 @@ struct Batch {
   int64_t tmax = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // The observation of a group (a few blocks, one per solve) runs on a second stream beside the
+  // passes that follow it; each step joins it back before the step counter advances.
+  hipStream_t side = nullptr;
+  std::vector<hipEvent_t> obs_ev;   // per group: its row pass has finished
+  hipEvent_t join_ev = nullptr;
 
   hipStream_t st() const { return ctx->stream; }
+
+  void ensure_side() {
+    if (side) return;
+    QSF_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    obs_ev.assign(groups.size(), nullptr);
+    for (auto& e : obs_ev) QSF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    QSF_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+  }
 
   Batch(qsf_ctx* c, const qsf_grid_desc* descs, const qsf_field_desc* fields, int ns, const qsf_prop_desc& p)
       : ctx(c), pd(p) {
@@ -1806,6 +1819,13 @@ struct Batch {
   ~Batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      for (auto e : obs_ev)
+        if (e) (void)hipEventDestroy(e);
+      if (join_ev) (void)hipEventDestroy(join_ev);
+      (void)hipStreamDestroy(side);
+    }
   }
 
   int64_t nmax() const {
@@ -1837,7 +1857,8 @@ struct Batch {
   // one step of every active solve; the first step has no pending inverse / X / observation
   void enqueue_step(bool first) {
     launch_batch_ptables(prec, targs.as<BatchTableArgs>(), (int)solves.size(), nmax(), ctr.as<int64_t>(), st());
-    for (auto& G : groups) {
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      Group& G = groups[gi];
       Grid& L = *G.lead;
       // 1D: one pass [X_a, OBS, X_b, FWD, P, INV] (the state stays in X between steps);
       // 2D: row pass [INV, X_a, OBS, X_b, FWD] and column pass [FWD, P, INV]
@@ -1856,7 +1877,12 @@ struct Batch {
       }
       int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, tune_class(rp), L.n[ra], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
-      if (!first) launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), st());
+      if (!first) {
+        // reads only what the row pass just wrote (block partials, gathered border points)
+        QSF_HIP(hipEventRecord(obs_ev[gi], st()));
+        QSF_HIP(hipStreamWaitEvent(side, obs_ev[gi], 0));
+        launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), side);
+      }
       if (L.D == 1) continue;
       PassParams cp = L.params(0, L.psi.p, L.psi.p);
       cp.batch = G.entries.as<BatchEntry>();
@@ -1866,11 +1892,16 @@ struct Batch {
       k = launch_axis_pass(prec, cp, st(), ctx->variant_of(prec, tune_class(cp), L.n[0], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
     }
+    if (!first) {
+      QSF_HIP(hipEventRecord(join_ev, side));
+      QSF_HIP(hipStreamWaitEvent(st(), join_ev, 0));
+    }
     launch_step_increment(ctr.as<int64_t>(), st());
     QSF_HIP(hipGetLastError());
   }
 
   void run(int64_t k, bool use_graph) {
+    ensure_side();
     while (k > 0 && launched < tmax) {
       if (launched == 0) {
         enqueue_step(true);
