@@ -173,6 +173,12 @@ def run_c4(steps, warmup):
             gs[0].set_psi(ground_like_state(n))
             gs[0].autotune(5)
             gs.close()
+    # then each size group's passes on the batched step itself: a single grid's timings need not
+    # rank the variants as the batch does, and the variants of the 2048^2 group differ by up to
+    # a quarter
+    for prec in (T.FP32_MIXED, T.FP32_STATE):
+        descs_p, _ = c4_solves(prec)
+        hip.autotune_batch(ctx, descs_p, fields, pd, lambda d: ground_like_state(int(d.n[0])))
     progress("autotuned; building the batches")
     pts = sum(int(d.n[0]) * int(d.n[1]) for d in descs32)
 

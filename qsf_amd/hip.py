@@ -536,3 +536,57 @@ class Batch:
         while not self.run(chunk, use_graph):
             pass
         self.finish()
+
+
+def autotune_batch(ctx, descs, fields, pdesc, state_of, steps=4):
+    """Pick the kernel variants of every size group of a batch by timing that group's batched step
+    (graph replay) once per compiled variant: first the column pass, then the row pass with the
+    column's choice. The picks go into the context's tuning table under the keys the batch looks
+    up (precision, pass class, axis length, grid size). `state_of(desc)` gives a start state.
+    Returns {(precision, class, length, grid size): (variant, ms per step)}."""
+    import time
+    groups = {}
+    for i, d in enumerate(descs):
+        shape = tuple(int(d.n[a]) for a in range(d.dim))
+        groups.setdefault((d.dim, shape, d.precision), []).append(i)
+    chosen = {}
+    for (dim, shape, prec), idx in groups.items():
+        m = int(np.prod(shape))
+        sub_d = [descs[i] for i in idx]
+        sub_f = [fields[i] for i in idx]
+        # the outermost axis of a 2D solve is its column pass (class 2); the last axis its row pass
+        passes = ([(2, shape[0])] if dim == 2 else []) + [(1, shape[-1])]
+        current = {cls: v for (pr, cls, n, mm, v, _ms) in ctx.tune_table()
+                   for c2, n2 in passes if pr == prec and mm == m and cls == c2 and n == n2}
+
+        def step_ms():
+            b = Batch(ctx, sub_d, sub_f, pdesc)
+            try:
+                for s, d in enumerate(sub_d):
+                    b.set_psi(s, state_of(d))
+                b.start()
+                b.run(2, use_graph=True)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                b.run(steps, use_graph=True)
+                ctx.synchronize()
+                return (time.perf_counter() - t0) / steps * 1e3
+            finally:
+                b.close()
+
+        for cls, n in passes:
+            for c2, n2 in passes:   # the other pass at its current pick
+                ctx.tune_set(prec, c2, n2, m, current.get(c2, 0))
+            best = None
+            for v in range(6):
+                try:
+                    ctx.tune_set(prec, cls, n, m, v)
+                except QSFError:
+                    break           # lengths without tuned variants have variant 0 only
+                ms = step_ms()
+                if best is None or ms < best[1]:
+                    best = (v, ms)
+            current[cls] = best[0]
+            ctx.tune_set(prec, cls, n, m, best[0])
+            chosen[(prec, cls, n, m)] = best
+    return chosen
