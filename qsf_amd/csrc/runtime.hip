@@ -1695,8 +1695,17 @@ struct Batch {
   hipStream_t side = nullptr;
   std::vector<hipEvent_t> obs_ev;   // per group: its row pass has finished
   hipEvent_t join_ev = nullptr;
+  // Size groups share nothing within a step but the phase tables and the step counter, so their
+  // passes run on kLanes streams (lane 0 is the context stream): forked after the table launch,
+  // joined before the counter advances. Groups go to lanes largest first, each to the lane with
+  // the fewest grid points so far, so the small groups fill the tails of the large ones.
+  static constexpr int kLanes = 3;
+  hipStream_t lane_st[kLanes] = {};
+  hipEvent_t lane_ev[kLanes] = {};   // lane 0: the fork; lanes 1..: their last pass has finished
+  std::vector<int> lane_of;          // per group
 
   hipStream_t st() const { return ctx->stream; }
+  hipStream_t lane(int gi) const { return lane_of[gi] == 0 ? st() : lane_st[lane_of[gi]]; }
 
   void ensure_side() {
     if (side) return;
@@ -1704,6 +1713,27 @@ struct Batch {
     obs_ev.assign(groups.size(), nullptr);
     for (auto& e : obs_ev) QSF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     QSF_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+    for (int l = 0; l < kLanes; ++l) {
+      if (l > 0) QSF_HIP(hipStreamCreateWithFlags(&lane_st[l], hipStreamNonBlocking));
+      QSF_HIP(hipEventCreateWithFlags(&lane_ev[l], hipEventDisableTiming));
+    }
+    std::vector<int> order(groups.size());
+    std::vector<double> work(groups.size());
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const Grid& L = *groups[gi].lead;
+      work[gi] = (double)groups[gi].members.size() * (double)L.n[0] * (double)(L.D > 1 ? L.n[1] : 1);
+      order[gi] = (int)gi;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    lane_of.assign(groups.size(), 0);
+    double load[kLanes] = {};
+    for (int gi : order) {
+      int best = 0;
+      for (int l = 1; l < kLanes; ++l)
+        if (load[l] < load[best]) best = l;
+      lane_of[gi] = best;
+      load[best] += work[gi];
+    }
   }
 
   Batch(qsf_ctx* c, const qsf_grid_desc* descs, const qsf_field_desc* fields, int ns, const qsf_prop_desc& p)
@@ -1825,6 +1855,13 @@ struct Batch {
         if (e) (void)hipEventDestroy(e);
       if (join_ev) (void)hipEventDestroy(join_ev);
       (void)hipStreamDestroy(side);
+      for (int l = 0; l < kLanes; ++l) {
+        if (lane_st[l]) {
+          (void)hipStreamSynchronize(lane_st[l]);
+          (void)hipStreamDestroy(lane_st[l]);
+        }
+        if (lane_ev[l]) (void)hipEventDestroy(lane_ev[l]);
+      }
     }
   }
 
@@ -1857,9 +1894,15 @@ struct Batch {
   // one step of every active solve; the first step has no pending inverse / X / observation
   void enqueue_step(bool first) {
     launch_batch_ptables(prec, targs.as<BatchTableArgs>(), (int)solves.size(), nmax(), ctr.as<int64_t>(), st());
+    bool used[kLanes] = {};
+    for (size_t gi = 0; gi < groups.size(); ++gi) used[lane_of[gi]] = true;
+    QSF_HIP(hipEventRecord(lane_ev[0], st()));
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) QSF_HIP(hipStreamWaitEvent(lane_st[l], lane_ev[0], 0));
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       Group& G = groups[gi];
       Grid& L = *G.lead;
+      const hipStream_t ls = lane((int)gi);
       // 1D: one pass [X_a, OBS, X_b, FWD, P, INV] (the state stays in X between steps);
       // 2D: row pass [INV, X_a, OBS, X_b, FWD] and column pass [FWD, P, INV]
       const int ra = L.D - 1;
@@ -1875,11 +1918,11 @@ struct Batch {
         L.obs_params(rp.obs, false);
         if (pd.nborders > 0) L.set_gather(rp.obs);
       }
-      int k = launch_axis_pass(prec, rp, st(), ctx->variant_of(prec, tune_class(rp), L.n[ra], L.m));
+      int k = launch_axis_pass(prec, rp, ls, ctx->variant_of(prec, tune_class(rp), L.n[ra], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
       if (!first) {
         // reads only what the row pass just wrote (block partials, gathered border points)
-        QSF_HIP(hipEventRecord(obs_ev[gi], st()));
+        QSF_HIP(hipEventRecord(obs_ev[gi], ls));
         QSF_HIP(hipStreamWaitEvent(side, obs_ev[gi], 0));
         launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), side);
       }
@@ -1889,9 +1932,14 @@ struct Batch {
       cp.step_ctr = ctr.as<int64_t>();
       cp.nbatch = (int32_t)G.members.size();
       cp.flags = F_FWD | F_P | F_INV2;
-      k = launch_axis_pass(prec, cp, st(), ctx->variant_of(prec, tune_class(cp), L.n[0], L.m));
+      k = launch_axis_pass(prec, cp, ls, ctx->variant_of(prec, tune_class(cp), L.n[0], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
     }
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) {
+        QSF_HIP(hipEventRecord(lane_ev[l], lane_st[l]));
+        QSF_HIP(hipStreamWaitEvent(st(), lane_ev[l], 0));
+      }
     if (!first) {
       QSF_HIP(hipEventRecord(join_ev, side));
       QSF_HIP(hipStreamWaitEvent(st(), join_ev, 0));
