@@ -1696,10 +1696,12 @@ struct Batch {
   std::vector<hipEvent_t> obs_ev;   // per group: its row pass has finished
   hipEvent_t join_ev = nullptr;
   // Size groups share nothing within a step but the phase tables and the step counter, so their
-  // passes run on kLanes streams (lane 0 is the context stream): forked after the table launch,
-  // joined before the counter advances. Groups go to lanes largest first, each to the lane with
-  // the fewest grid points so far, so the small groups fill the tails of the large ones.
-  static constexpr int kLanes = 3;
+  // passes run on nlanes streams (lane 0 is the context stream; QSF_BATCH_STREAMS = 1..4, default
+  // 3): forked after the table launch, joined before the counter advances. Groups go to lanes
+  // largest first, each to the lane with the fewest grid points so far, so the small groups fill
+  // the tails of the large ones.
+  static constexpr int kLanes = 4;
+  int nlanes = 3;
   hipStream_t lane_st[kLanes] = {};
   hipEvent_t lane_ev[kLanes] = {};   // lane 0: the fork; lanes 1..: their last pass has finished
   std::vector<int> lane_of;          // per group
@@ -1713,7 +1715,8 @@ struct Batch {
     obs_ev.assign(groups.size(), nullptr);
     for (auto& e : obs_ev) QSF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     QSF_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
-    for (int l = 0; l < kLanes; ++l) {
+    if (const char* e = std::getenv("QSF_BATCH_STREAMS")) nlanes = std::max(1, std::min(kLanes, std::atoi(e)));
+    for (int l = 0; l < nlanes; ++l) {
       if (l > 0) QSF_HIP(hipStreamCreateWithFlags(&lane_st[l], hipStreamNonBlocking));
       QSF_HIP(hipEventCreateWithFlags(&lane_ev[l], hipEventDisableTiming));
     }
@@ -1729,7 +1732,7 @@ struct Batch {
     double load[kLanes] = {};
     for (int gi : order) {
       int best = 0;
-      for (int l = 1; l < kLanes; ++l)
+      for (int l = 1; l < nlanes; ++l)
         if (load[l] < load[best]) best = l;
       lane_of[gi] = best;
       load[best] += work[gi];
