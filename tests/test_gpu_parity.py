@@ -478,6 +478,39 @@ def test_batch_mixed_dimensions(ctx, prec):
     b.close()
 
 
+@pytest.mark.parametrize("prec", [T.FP64, T.FP32_MIXED])
+def test_batch_stream_count_bit_identical(ctx, prec, monkeypatch):
+    # size groups share nothing within a step but the P tables and the step counter, so running
+    # them on one, three or four streams (QSF_BATCH_STREAMS) gives the same bits: states and
+    # record rows of every solve, under hipGraph replay
+    from qsf_amd import hip
+    kw2 = dict(potential=T.POT_EBERLY, Ncharge=2, Echarge=-1, Nsoft=0.5, Esoft=0.33,
+               absorber=T.ABS_CAP, nCAP=8)
+    kw1 = dict(potential=T.POT_COULOMB, Ncharge=1, Echarge=-1, Nsoft=1.0, absorber=T.ABS_CAP, nCAP=30)
+    specs = [(2, 128, kw2, 0.3), (2, 64, kw2, 0.3), (2, 96, kw2, 0.3), (1, 256, kw1, 0.1),
+             (2, 80, kw2, 0.3), (2, 128, kw2, 0.3), (2, 48, kw2, 0.3), (1, 384, kw1, 0.1)]
+    descs = [T.grid_desc(dim, n, dx, precision=prec, **kw) for dim, n, kw, dx in specs]
+    fields = [T.field_desc(T.AXIS_X if dim == 1 else T.AXIS_XY, 0.05 + 0.01 * i, 2.0, 1 + (i % 2),
+                           phase_in_pi=0.25 * i) for i, (dim, n, kw, dx) in enumerate(specs)]
+    pdesc = T.prop_desc(T.MODE_RE, 1, 0.05, gauge=T.GAUGE_VELOCITY, outputs=T.OUT_NORM | T.OUT_FLUX,
+                        borders=[(T.BORDER_BOX, 12)])
+    psis = [crand(n ** dim) * 0.01 for dim, n, kw, dx in specs]
+    results = {}
+    for streams in (1, 3, 4):
+        print(f"[case] batch on {streams} stream(s), prec={prec}", flush=True)
+        monkeypatch.setenv("QSF_BATCH_STREAMS", str(streams))
+        b = hip.Batch(ctx, descs, fields, pdesc)
+        for s, p in enumerate(psis):
+            b.set_psi(s, p)
+        b.run_all(use_graph=True)
+        results[streams] = [(b.psi(s), b.records(s)) for s in range(len(specs))]
+        b.close()
+    for streams in (3, 4):
+        for s in range(len(specs)):
+            assert np.array_equal(results[streams][s][0], results[1][s][0])
+            assert np.array_equal(results[streams][s][1], results[1][s][1], equal_nan=True)
+
+
 def test_cos18_edge_node_stays_finite(ctx):
     """cos^(1/8) with a width whose π/2·node/nCAP rounds past π/2 at the edge node (nCAP=13):
     the mask there is exactly 0 on both sides and the propagation stays finite."""
