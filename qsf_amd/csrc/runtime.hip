@@ -895,7 +895,10 @@ struct Grid {
     launch_flux(f, st());
   }
 
-  void flux(const double* vgC, const qsf_border_desc* b, int nb, double* out) {
+  // flux of each border family into out (host); with accum, also accum[b] += flux * w on the
+  // device (the propagator's FP64 flux time integral)
+  void flux(const double* vgC, const qsf_border_desc* b, int nb, double* out, double* accum = nullptr,
+            double w = 0.0) {
     QSF_REQUIRE(rep == QSF_REP_X, QSF_ERR_STATE, "flux needs the X representation");
     build_gather(b, nb);
     PassParams p = params(D - 1, psi.p, psi.p);
@@ -903,7 +906,7 @@ struct Grid {
     obs_params(p.obs, false);
     set_gather(p.obs);
     run(p);
-    flux_from_gather(vgC, nullptr, nullptr, 0.0);
+    flux_from_gather(vgC, nullptr, accum, w);
     QSF_HIP(hipMemcpyAsync(out, gather->out.p, nb * sizeof(double), hipMemcpyDeviceToHost, st()));
     QSF_HIP(hipStreamSynchronize(st()));
   }
@@ -1529,12 +1532,8 @@ struct Prop {
       if ((o & QSF_OUT_FLUX) && pd.nborders > 0) {
         double C[3];
         coup.values(C);
-        g.flux(couples() == QSF_REP_P ? C : nullptr, pd.borders, pd.nborders, &row[7]);
-        // time integral of the flux (device copy kept in step with the fused path)
-        std::vector<double> acc(16);
-        QSF_HIP(hipMemcpy(acc.data(), fluxint.p, 16 * sizeof(double), hipMemcpyDeviceToHost));
-        for (int b = 0; b < pd.nborders; ++b) acc[b] += row[7 + b] * dt;
-        QSF_HIP(hipMemcpy(fluxint.p, acc.data(), 16 * sizeof(double), hipMemcpyHostToDevice));
+        // the flux time integral accumulates on the device, as on the fused path
+        g.flux(couples() == QSF_REP_P ? C : nullptr, pd.borders, pd.nborders, &row[7], fluxint.as<double>(), dt);
       }
       if (o & QSF_OUT_EKIN) {
         g.fourier(QSF_REP_P);

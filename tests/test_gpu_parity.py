@@ -214,6 +214,17 @@ def run_both(ctx, desc, pdesc, fields, psi0, limit, exec_mode=0):
     return dev, ref, pd, po, rd, ro
 
 
+def check_flux_integral(pd, rd, ro, dt, rtol, atol):
+    # the device's FP64 flux time integral: every record row (the start row included) adds
+    # flux * dt, on the fused and the member-by-member path alike. Against the device's own rows
+    # up to summation order, and against the oracle's rows within the row tolerance (rtol, atol)
+    got = pd.flux_integral()
+    fd, fo = rd[:, T.REC_FLUX0:], ro[:, T.REC_FLUX0:]
+    assert got.shape == (fd.shape[1],)
+    assert np.allclose(got, fd.sum(axis=0) * dt, rtol=1e-12, atol=1e-15 * np.abs(fd).sum() * dt)
+    assert np.allclose(got, fo.sum(axis=0) * dt, rtol=rtol, atol=atol * dt * fo.shape[0])
+
+
 @pytest.mark.parametrize("exec_mode", [0, 1])
 def test_propagation_1d_length_gauge(ctx, exec_mode):
     n, dx = 512, 0.1
@@ -231,6 +242,7 @@ def test_propagation_1d_length_gauge(ctx, exec_mode):
     assert np.allclose(rd[:, :2], ro[:, :2], rtol=0, atol=1e-12)   # step, time
     assert np.allclose(rd[:, 2:6], ro[:, 2:6], rtol=1e-11, atol=1e-13)
     assert np.allclose(rd[:, 7:], ro[:, 7:], rtol=1e-9, atol=1e-13)
+    check_flux_integral(pd, rd, ro, 0.05, 1e-9, 1e-13)
 
 
 @pytest.mark.parametrize("exec_mode", [0, 1])
@@ -247,6 +259,7 @@ def test_propagation_2d_velocity_gauge_order2(ctx, exec_mode):
     assert rel(dev[0].psi(), ref[0].psi()) < FP64_TOL
     assert np.allclose(rd[:, 2:4], ro[:, 2:4], rtol=1e-11)
     assert np.allclose(rd[:, 7:], ro[:, 7:], rtol=1e-8, atol=1e-12 * np.abs(ro[:, 7:]).max())
+    check_flux_integral(pd, rd, ro, 0.02, 1e-8, 1e-12 * np.abs(ro[:, 7:]).max())
 
 
 @pytest.mark.parametrize("exec_mode,n", [(0, 32), (1, 32), (0, 64)])
