@@ -1689,36 +1689,30 @@ struct Batch {
   int64_t tmax = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  // The observation of a group (a few blocks, one per solve) runs on a second stream beside the
-  // passes that follow it; each step joins it back before the step counter advances.
-  hipStream_t side = nullptr;
-  std::vector<hipEvent_t> obs_ev;   // per group: its row pass has finished
-  hipEvent_t join_ev = nullptr;
   // Size groups share nothing within a step but the phase tables and the step counter, so their
-  // passes run on nlanes streams (lane 0 is the context stream; QSF_BATCH_STREAMS = 1..4, default
-  // 3): forked after the table launch, joined before the counter advances. Groups go to lanes
-  // largest first, each to the lane with the fewest grid points so far, so the small groups fill
-  // the tails of the large ones.
+  // passes and observations run on nlanes streams (lane 0 is the context stream;
+  // QSF_BATCH_STREAMS = 1..4, default 4, one per hardware queue of the process): forked after the
+  // table launch, joined before the counter advances. Groups go to lanes largest first, each to
+  // the lane with the fewest grid points so far, so the small groups fill the tails of the large
+  // ones.
   static constexpr int kLanes = 4;
-  int nlanes = 3;
+  int nlanes = 0;                    // 0 until the lanes are made
   hipStream_t lane_st[kLanes] = {};
-  hipEvent_t lane_ev[kLanes] = {};   // lane 0: the fork; lanes 1..: their last pass has finished
+  hipEvent_t lane_ev[kLanes] = {};   // lane 0: the fork; lanes 1..: their last launch has finished
   std::vector<int> lane_of;          // per group
 
   hipStream_t st() const { return ctx->stream; }
   hipStream_t lane(int gi) const { return lane_of[gi] == 0 ? st() : lane_st[lane_of[gi]]; }
 
-  void ensure_side() {
-    if (side) return;
-    QSF_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    obs_ev.assign(groups.size(), nullptr);
-    for (auto& e : obs_ev) QSF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    QSF_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
-    if (const char* e = std::getenv("QSF_BATCH_STREAMS")) nlanes = std::max(1, std::min(kLanes, std::atoi(e)));
-    for (int l = 0; l < nlanes; ++l) {
+  void ensure_lanes() {
+    if (nlanes) return;
+    int want = kLanes;
+    if (const char* e = std::getenv("QSF_BATCH_STREAMS")) want = std::max(1, std::min(kLanes, std::atoi(e)));
+    for (int l = 0; l < want; ++l) {
       if (l > 0) QSF_HIP(hipStreamCreateWithFlags(&lane_st[l], hipStreamNonBlocking));
       QSF_HIP(hipEventCreateWithFlags(&lane_ev[l], hipEventDisableTiming));
     }
+    nlanes = want;
     std::vector<int> order(groups.size());
     std::vector<double> work(groups.size());
     for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -1851,19 +1845,12 @@ struct Batch {
   ~Batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (side) {
-      (void)hipStreamSynchronize(side);
-      for (auto e : obs_ev)
-        if (e) (void)hipEventDestroy(e);
-      if (join_ev) (void)hipEventDestroy(join_ev);
-      (void)hipStreamDestroy(side);
-      for (int l = 0; l < kLanes; ++l) {
-        if (lane_st[l]) {
-          (void)hipStreamSynchronize(lane_st[l]);
-          (void)hipStreamDestroy(lane_st[l]);
-        }
-        if (lane_ev[l]) (void)hipEventDestroy(lane_ev[l]);
+    for (int l = 0; l < kLanes; ++l) {
+      if (lane_st[l]) {
+        (void)hipStreamSynchronize(lane_st[l]);
+        (void)hipStreamDestroy(lane_st[l]);
       }
+      if (lane_ev[l]) (void)hipEventDestroy(lane_ev[l]);
     }
   }
 
@@ -1922,12 +1909,8 @@ struct Batch {
       }
       int k = launch_axis_pass(prec, rp, ls, ctx->variant_of(prec, tune_class(rp), L.n[ra], L.m));
       QSF_REQUIRE(k >= 0, QSF_ERR_UNSUPPORTED, "no FFT kernel for this axis length");
-      if (!first) {
-        // reads only what the row pass just wrote (block partials, gathered border points)
-        QSF_HIP(hipEventRecord(obs_ev[gi], ls));
-        QSF_HIP(hipStreamWaitEvent(side, obs_ev[gi], 0));
-        launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), side);
-      }
+      // reads only what the row pass just wrote (block partials, gathered border points)
+      if (!first) launch_batch_obs(G.oargs.as<BatchObsArgs>(), (int)G.members.size(), G.geo, ctr.as<int64_t>(), ls);
       if (L.D == 1) continue;
       PassParams cp = L.params(0, L.psi.p, L.psi.p);
       cp.batch = G.entries.as<BatchEntry>();
@@ -1942,16 +1925,12 @@ struct Batch {
         QSF_HIP(hipEventRecord(lane_ev[l], lane_st[l]));
         QSF_HIP(hipStreamWaitEvent(st(), lane_ev[l], 0));
       }
-    if (!first) {
-      QSF_HIP(hipEventRecord(join_ev, side));
-      QSF_HIP(hipStreamWaitEvent(st(), join_ev, 0));
-    }
     launch_step_increment(ctr.as<int64_t>(), st());
     QSF_HIP(hipGetLastError());
   }
 
   void run(int64_t k, bool use_graph) {
-    ensure_side();
+    ensure_lanes();
     while (k > 0 && launched < tmax) {
       if (launched == 0) {
         enqueue_step(true);
