@@ -160,8 +160,8 @@ int qsf_prop_set_exec(qsf_prop* p, int exec_mode);
  *      device step counter; with use_graph it is captured once and replayed as a hipGraph.
  *      Each solve follows its own SplitPropagator loop (max_steps+2 steps, propagator.h:153,
  *      443-451); finished solves drop out of the launches. Within a step the size groups and
- *      their observations run on streams of the batch's own, forked from the context stream
- *      and joined back into it before the step ends: work queued on the context stream after
+ *      their observations run on the context stream and streams of the batch's own, forked from
+ *      the context stream and joined back into it before the step ends: work queued on it after
  *      qsf_batch_run sees every step complete. ---- */
 typedef struct qsf_batch qsf_batch;
 int qsf_batch_create(qsf_ctx* ctx, const qsf_grid_desc* grids, const qsf_field_desc* fields, int nsolves,
