@@ -538,7 +538,7 @@ class Batch:
         self.finish()
 
 
-def autotune_batch(ctx, descs, fields, pdesc, state_of, steps=4):
+def autotune_batch(ctx, descs, fields, pdesc, state_of, steps=8):
     """Pick the kernel variants of every size group of a batch by timing that group's batched step
     (graph replay) once per compiled variant: first the column pass, then the row pass with the
     column's choice. The picks go into the context's tuning table under the keys the batch looks
