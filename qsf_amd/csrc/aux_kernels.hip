@@ -160,9 +160,9 @@ void launch_roots(int precision, void* out, int64_t n, hipStream_t s) {
 }
 
 // One block, fixed summation order: thread k sums blocks k, k+256, ... then a tree.
-__global__ void reduce_partials_kernel(const double* partials, int64_t count, int width, double* out) {
+__global__ void reduce_partials_kernel(const double* partials, int64_t count, int width, int ncols, double* out) {
   __shared__ double red[256];
-  for (int c = 0; c < width; ++c) {
+  for (int c = 0; c < ncols; ++c) {
     double acc = 0.0;
     for (int64_t k = threadIdx.x; k < count; k += 256) acc += partials[k * width + c];
     red[threadIdx.x] = acc;
@@ -177,10 +177,11 @@ __global__ void reduce_partials_kernel(const double* partials, int64_t count, in
 }
 
 // First level: block b sums the contiguous chunk [b*chunk, (b+1)*chunk) of the partials.
-__global__ void reduce_chunks_kernel(const double* partials, int64_t count, int width, int64_t chunk, double* out) {
+__global__ void reduce_chunks_kernel(const double* partials, int64_t count, int width, int ncols, int64_t chunk,
+                                     double* out) {
   __shared__ double red[256];
   const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < count ? lo + chunk : count;
-  for (int c = 0; c < width; ++c) {
+  for (int c = 0; c < ncols; ++c) {
     double acc = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += 256) acc += partials[k * width + c];
     red[threadIdx.x] = acc;
@@ -189,21 +190,22 @@ __global__ void reduce_chunks_kernel(const double* partials, int64_t count, int 
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       block_sync();
     }
-    if (threadIdx.x == 0) out[blockIdx.x * width + c] = red[0];
+    if (threadIdx.x == 0) out[blockIdx.x * ncols + c] = red[0];
     block_sync();
   }
 }
 
 void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, double* scratch,
-                            hipStream_t s) {
+                            hipStream_t s, int ncols) {
+  if (ncols < 0) ncols = width;
   if (count <= 4096 || !scratch) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, count, width, out);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partials, count, width, ncols, out);
     return;
   }
-  // two fixed-order levels: 256 chunk sums into `scratch` (256*width doubles), then one block
+  // two fixed-order levels: 256 chunk sums into `scratch` (256*ncols doubles), then one block
   const int64_t chunk = (count + 255) / 256;
-  hipLaunchKernelGGL(reduce_chunks_kernel, dim3(256), dim3(256), 0, s, partials, count, width, chunk, scratch);
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, scratch, (int64_t)256, width, out);
+  hipLaunchKernelGGL(reduce_chunks_kernel, dim3(256), dim3(256), 0, s, partials, count, width, ncols, chunk, scratch);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, scratch, (int64_t)256, ncols, ncols, out);
 }
 
 template <typename Real>

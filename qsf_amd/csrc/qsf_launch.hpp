@@ -86,9 +86,11 @@ void launch_radial_x(const RadialArgs& a, hipStream_t s);
 void launch_roots(int precision, void* out, int64_t n, hipStream_t s);
 
 // Sum of `count` blocks of `width` doubles into out[width] (fixed order). Large counts use a
-// second level through `scratch` (256*width doubles).
+// second level through `scratch` (256*width doubles). With ncols >= 0 only the first ncols
+// doubles of each block are summed, into out[ncols] (a column reduced on its own: pass the
+// partials from that column on).
 void launch_reduce_partials(const double* partials, int64_t count, int width, double* out, double* scratch,
-                            hipStream_t s);
+                            hipStream_t s, int ncols = -1);
 
 // Flux through border families from gathered boundary values.
 struct FluxArgs {

@@ -1382,9 +1382,8 @@ struct Prop {
       int64_t maxn = *std::max_element(gg.n, gg.n + 3);
       for (int a = 0; a < gg.D; ++a) q.obs.kin_w[a] = gg.kinw.as<double>() + a * maxn;
       int t2 = gg.run(q);
-      DBuf& tmp = gg.red;
-      launch_reduce_partials(gg.partials.as<double>(), t2, 3, tmp.as<double>(), gg.redscratch.as<double>(), st());
-      QSF_HIP(hipMemcpyAsync(rec + 2, tmp.as<double>() + 2, sizeof(double), hipMemcpyDeviceToDevice, st()));
+      // the kinetic column alone, straight into the record row (its norm and epot stay)
+      launch_reduce_partials(gg.partials.as<double>() + 2, t2, 3, rec + 2, gg.redscratch.as<double>(), st(), 1);
     }
   }
 
